@@ -1,0 +1,694 @@
+// pj_api.cpp -- host orchestration and the extern "C" boundary of libpj.so.
+//
+// The reference drives its phases from main.cu:759-893 with blocking copies and
+// per-phase event synchronisation; here one stream-ordered sequence of kernels
+// runs with no host round trip, on caller-provided workspace.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "../../include/pj.h"
+#include "pj_device.h"
+#include "pj_internal.h"
+#include "pj_misc.h"
+
+namespace pj {
+
+static thread_local std::string g_err;
+
+static pj_status fail(pj_status st, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return st;
+}
+
+#define PJ_HIP(call)                                                                             \
+  do {                                                                                           \
+    hipError_t e_ = (call);                                                                      \
+    if (e_ != hipSuccess) return fail(PJ_ERR_HIP, "%s failed: %s", #call, hipGetErrorString(e_)); \
+  } while (0)
+
+int tuple_bytes(int key_bytes) { return key_bytes == 4 ? 8 : 16; }
+
+static int scatter_tile(int key_bytes) { return key_bytes == 4 ? kScatterTile4 : kScatterTile8; }
+
+uint32_t choose_nseg(int64_t n) {
+  int64_t s = n / (int64_t(kNumXcd) * 2 * kScatterTile4);
+  if (s < 1) s = 1;
+  if (s > kMaxSegPerXcd) s = kMaxSegPerXcd;
+  return uint32_t(s) * kNumXcd;
+}
+
+static size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+
+struct Carver {
+  char* base;
+  size_t off = 0;
+  void* take(size_t bytes) {
+    void* p = base ? base + off : nullptr;
+    off += align256(bytes);
+    return p;
+  }
+};
+
+static void carve_table(Carver& c, int64_t n, int key_bytes, const PartParams& pp, bool two_pass, PartTable* T) {
+  const size_t tb = size_t(tuple_bytes(key_bytes));
+  T->n = uint32_t(n);
+  T->nseg = choose_nseg(n);
+  T->buf1 = c.take(size_t(n) * tb + 64);
+  T->buf2 = two_pass ? c.take(size_t(n) * tb + 64) : nullptr;
+  T->mat = static_cast<uint32_t*>(c.take(size_t(T->nseg) * pp.P * 4));
+  T->matc = two_pass ? static_cast<uint32_t*>(c.take(size_t(T->nseg) * pp.F1 * 4)) : nullptr;
+  T->tot = static_cast<uint32_t*>(c.take(size_t(pp.P) * 4));
+  T->totc = two_pass ? static_cast<uint32_t*>(c.take(size_t(pp.F1) * 4)) : nullptr;
+  T->off = static_cast<uint32_t*>(c.take(size_t(pp.P + 1) * 4));
+}
+
+static void carve_join(Carver& c, const PartParams& pp, Layout* L) {
+  L->item_scan = static_cast<uint64_t*>(c.take(size_t(pp.P + 1) * 8));
+  L->chunk_ctr = static_cast<uint32_t*>(c.take(64));
+  L->dstate = static_cast<DefragState*>(c.take(sizeof(DefragState)));
+  L->wg_last = static_cast<uint64_t*>(c.take(size_t(kJoinGrid) * 8));
+  L->hole_lo = static_cast<uint64_t*>(c.take((size_t(kJoinGrid) + 3) * 8));
+  L->hole_pre = static_cast<uint64_t*>(c.take((size_t(kJoinGrid) + 3) * 8));
+  L->stray_lo = static_cast<uint64_t*>(c.take((size_t(kJoinGrid) + 3) * 8));
+  L->stray_pre = static_cast<uint64_t*>(c.take((size_t(kJoinGrid) + 3) * 8));
+  L->spill = static_cast<pj_pair*>(c.take(size_t(kJoinGrid) * size_t(kOutChunk) * 8));
+}
+
+size_t layout_join(int64_t nR, int64_t nS, int key_bytes, const PartParams& pp, char* base, Layout* L) {
+  Carver c{base};
+  Layout tmp{};
+  Layout* Lp = L ? L : &tmp;
+  Lp->ntables = 2;
+  Lp->pp = pp;
+  carve_table(c, nR, key_bytes, pp, pp.passes == 2, &Lp->t[0]);
+  carve_table(c, nS, key_bytes, pp, pp.passes == 2, &Lp->t[1]);
+  carve_join(c, pp, Lp);
+  c.take(64);  // device pair count of the synchronous entry point
+  Lp->bytes = c.off;
+  return c.off;
+}
+
+// ---------------------------------------------------------------------------
+// Validation and plans
+// ---------------------------------------------------------------------------
+static pj_status check_table(const pj_table* t, const char* name, bool need_data = true) {
+  if (!t) return fail(PJ_ERR_INVALID_ARG, "%s table is NULL", name);
+  if (t->key_bytes != 4 && t->key_bytes != 8)
+    return fail(PJ_ERR_INVALID_ARG, "%s key_bytes must be 4 or 8 (got %d)", name, t->key_bytes);
+  if (t->n < 0 || t->n >= (int64_t(1) << 31))
+    return fail(PJ_ERR_INVALID_ARG, "%s row count %lld outside [0, 2^31)", name, (long long)t->n);
+  if (need_data && t->n > 0 && !t->keys) return fail(PJ_ERR_INVALID_ARG, "%s keys pointer is NULL", name);
+  return PJ_OK;
+}
+
+// need_data = false for the planning functions, which read only sizes and widths.
+static pj_status check_pair(const pj_table* b, const pj_table* p, bool need_data = true) {
+  pj_status st = check_table(b, "build", need_data);
+  if (st != PJ_OK) return st;
+  st = check_table(p, "probe", need_data);
+  if (st != PJ_OK) return st;
+  if (b->key_bytes != p->key_bytes)
+    return fail(PJ_ERR_INVALID_ARG, "build and probe key widths differ (%d vs %d)", b->key_bytes, p->key_bytes);
+  return PJ_OK;
+}
+
+static void plain_plan(int64_t nR, int key_bytes, pj_plan* plan) {
+  std::memset(plan, 0, sizeof(*plan));
+  const double f = 3.0;
+  const double cap = double(join_table_capacity(key_bytes) / 2 - 1);
+  const double target = cap < 3072.0 ? cap : 3072.0;  // build tuples per chunk at factor f
+  int B = 1;
+  while (B < kMaxTotalBits && f * double(nR) / double(int64_t(1) << B) > target) ++B;
+  plan->max_partition_size_factor = f;
+  plan->unique_build_keys = 0;
+  plan->probe_slice = 0;
+  plan->source = 0;
+  if (B <= 11) {
+    plan->passes = 1;
+    plan->bits1 = B;
+    plan->bits2 = 0;
+  } else {
+    plan->passes = 2;
+    plan->bits1 = (B + 1) / 2;
+    plan->bits2 = B / 2;
+  }
+}
+
+static pj_status check_plan(const pj_plan* plan) {
+  if (plan->passes != 1 && plan->passes != 2)
+    return fail(PJ_ERR_INVALID_ARG, "plan.passes must be 1 or 2 (got %d)", plan->passes);
+  if (plan->bits1 < 1 || plan->bits1 > 12)
+    return fail(PJ_ERR_INVALID_ARG, "plan.bits1 must be in [1, 12] (got %d)", plan->bits1);
+  if (plan->passes == 2 && (plan->bits2 < 1 || plan->bits2 > 12))
+    return fail(PJ_ERR_INVALID_ARG, "plan.bits2 must be in [1, 12] for two passes (got %d)", plan->bits2);
+  if (plan->passes == 1 && plan->bits2 != 0)
+    return fail(PJ_ERR_INVALID_ARG, "plan.bits2 must be 0 for one pass (got %d)", plan->bits2);
+  if (plan->bits1 + plan->bits2 > kMaxTotalBits)
+    return fail(PJ_ERR_UNSUPPORTED, "bits1 + bits2 = %d exceeds %d", plan->bits1 + plan->bits2, kMaxTotalBits);
+  if (!(plan->max_partition_size_factor > 0.0))
+    return fail(PJ_ERR_INVALID_ARG, "plan.max_partition_size_factor must be > 0");
+  if (plan->probe_slice < 0) return fail(PJ_ERR_INVALID_ARG, "plan.probe_slice must be >= 0");
+  return PJ_OK;
+}
+
+static PartParams params_of(const pj_plan* plan) {
+  PartParams pp{};
+  pp.passes = plan->passes;
+  pp.b1 = plan->bits1;
+  pp.b2 = plan->passes == 2 ? plan->bits2 : 0;
+  pp.B = pp.b1 + pp.b2;
+  pp.P = 1 << pp.B;
+  pp.F1 = 1 << pp.b1;
+  pp.F2 = 1 << pp.b2;
+  pp.shift0 = 0;
+  return pp;
+}
+
+// base = ceil(f * |R| / P) (main.cu:867), clamped to the LDS table.
+static int join_base(int64_t nR, int P, double f, int key_bytes) {
+  double b = std::ceil(f * double(nR) / double(P));
+  const int cap = join_table_capacity(key_bytes) / 2 - 1;
+  if (b < 1.0) b = 1.0;
+  if (b > double(cap)) b = double(cap);
+  return int(b);
+}
+
+static int64_t join_slice(const pj_plan* plan, int base) {
+  if (plan->probe_slice > 0) return plan->probe_slice;
+  int64_t s = int64_t(base) * 8;
+  return s < 16384 ? 16384 : s;
+}
+
+// ---------------------------------------------------------------------------
+// Profiling
+// ---------------------------------------------------------------------------
+struct Prof {
+  bool on = false;
+  bool recorded = false;
+  hipEvent_t ev[9] = {};
+  bool created = false;
+};
+static thread_local Prof g_prof;
+
+static void prof_mark(int i, hipStream_t s) {
+  if (!g_prof.on || !g_prof.created) return;
+  (void)hipEventRecord(g_prof.ev[i], s);
+}
+
+// ---------------------------------------------------------------------------
+// The join pipeline
+// ---------------------------------------------------------------------------
+static pj_status run_join(const pj_table* build, const pj_table* probe, const pj_plan* plan_in, pj_pair* out,
+                          int64_t capacity, int64_t* d_count, void* workspace, size_t ws_bytes, hipStream_t s) {
+  pj_status st = check_pair(build, probe);
+  if (st != PJ_OK) return st;
+  if (capacity < 0) return fail(PJ_ERR_INVALID_ARG, "out_capacity must be >= 0");
+  if (capacity > 0 && !out) return fail(PJ_ERR_INVALID_ARG, "out is NULL with out_capacity > 0");
+  if (!d_count) return fail(PJ_ERR_INVALID_ARG, "pair count pointer is NULL");
+  pj_plan plan;
+  if (plan_in) plan = *plan_in;
+  else plain_plan(build->n, build->key_bytes, &plan);
+  st = check_plan(&plan);
+  if (st != PJ_OK) return st;
+  const int kb = build->key_bytes;
+  if (build->n == 0 || probe->n == 0) {
+    PJ_HIP(launch_fill_u64(reinterpret_cast<uint64_t*>(d_count), 0, 1, s));
+    if (g_prof.on) g_prof.recorded = false;
+    return PJ_OK;
+  }
+  const PartParams pp = params_of(&plan);
+  const size_t need = layout_join(build->n, probe->n, kb, pp, nullptr, nullptr);
+  if (!workspace || ws_bytes < need)
+    return fail(PJ_ERR_WORKSPACE, "workspace of %zu bytes is smaller than the %zu required", ws_bytes, need);
+  if (reinterpret_cast<uintptr_t>(workspace) % 256 != 0)
+    return fail(PJ_ERR_INVALID_ARG, "workspace must be 256-byte aligned");
+  Layout L{};
+  layout_join(build->n, probe->n, kb, pp, static_cast<char*>(workspace), &L);
+  L.t[0].keys = build->keys;
+  L.t[0].pay = build->payload;
+  L.t[1].keys = probe->keys;
+  L.t[1].pay = probe->payload;
+  const int base = join_base(build->n, pp.P, plan.max_partition_size_factor, kb);
+  const int64_t slice = join_slice(&plan, base);
+
+  if (g_prof.on && !g_prof.created) {
+    for (auto& e : g_prof.ev) PJ_HIP(hipEventCreate(&e));
+    g_prof.created = true;
+  }
+  prof_mark(0, s);
+  PJ_HIP(launch_hist(L, kb, s));
+  prof_mark(1, s);
+  PJ_HIP(launch_colscan(L, s));
+  PJ_HIP(launch_offsets(L, s));
+  prof_mark(2, s);
+  PJ_HIP(launch_pass1(L, kb, s));
+  prof_mark(3, s);
+  if (pp.passes == 2) PJ_HIP(launch_pass2(L, kb, s));
+  prof_mark(4, s);
+  PJ_HIP(launch_join_plan(L, base, slice, s));
+  prof_mark(5, s);
+  PJ_HIP(launch_join(L, kb, base, slice, plan.unique_build_keys, out, capacity, s));
+  prof_mark(6, s);
+  PJ_HIP(launch_defrag(L, out, capacity, d_count, s));
+  prof_mark(7, s);
+  if (g_prof.on) g_prof.recorded = true;
+  return PJ_OK;
+}
+
+}  // namespace pj
+
+using namespace pj;
+
+extern "C" {
+
+const char* pj_version(void) { return "partitioned_join_amd 0.1 (gfx950)"; }
+
+const char* pj_last_error(void) { return g_err.c_str(); }
+
+pj_status pj_plan_default(const pj_table* build, const pj_table* probe, pj_plan* plan) {
+  pj_status st = check_pair(build, probe, false);
+  if (st != PJ_OK) return st;
+  if (!plan) return fail(PJ_ERR_INVALID_ARG, "plan is NULL");
+  plain_plan(build->n, build->key_bytes, plan);
+  return PJ_OK;
+}
+
+pj_status pj_workspace_size(const pj_table* build, const pj_table* probe, const pj_plan* plan_in, size_t* bytes) {
+  pj_status st = check_pair(build, probe, false);
+  if (st != PJ_OK) return st;
+  if (!bytes) return fail(PJ_ERR_INVALID_ARG, "bytes is NULL");
+  pj_plan plan;
+  if (plan_in) plan = *plan_in;
+  else plain_plan(build->n, build->key_bytes, &plan);
+  st = check_plan(&plan);
+  if (st != PJ_OK) return st;
+  *bytes = layout_join(build->n, probe->n, build->key_bytes, params_of(&plan), nullptr, nullptr);
+  return PJ_OK;
+}
+
+pj_status pj_partitioned_join_async(const pj_table* build, const pj_table* probe, const pj_plan* plan, pj_pair* out,
+                                    int64_t out_capacity, int64_t* d_out_count, void* workspace,
+                                    size_t workspace_bytes, hipStream_t stream) {
+  return run_join(build, probe, plan, out, out_capacity, d_out_count, workspace, workspace_bytes, stream);
+}
+
+pj_status pj_partitioned_join(const pj_table* build, const pj_table* probe, const pj_plan* plan, pj_pair* out,
+                              int64_t out_capacity, int64_t* out_count, void* workspace, size_t workspace_bytes,
+                              hipStream_t stream) {
+  if (!out_count) return fail(PJ_ERR_INVALID_ARG, "out_count is NULL");
+  pj_status st = check_pair(build, probe);
+  if (st != PJ_OK) return st;
+  // The device pair count lives in the last 64 bytes of the layout.
+  size_t need = 0;
+  st = pj_workspace_size(build, probe, plan, &need);
+  if (st != PJ_OK) return st;
+  if (!workspace || workspace_bytes < need)
+    return fail(PJ_ERR_WORKSPACE, "workspace of %zu bytes is smaller than the %zu required", workspace_bytes, need);
+  int64_t* d_count = reinterpret_cast<int64_t*>(static_cast<char*>(workspace) + need - 256);
+  st = run_join(build, probe, plan, out, out_capacity, d_count, workspace, workspace_bytes, stream);
+  if (st != PJ_OK) return st;
+  int64_t h = 0;
+  PJ_HIP(hipMemcpyAsync(&h, d_count, sizeof(h), hipMemcpyDeviceToHost, stream));
+  PJ_HIP(hipStreamSynchronize(stream));
+  *out_count = h;
+  if (h > out_capacity)
+    return fail(PJ_ERR_CAPACITY, "join produced %lld pairs, output capacity is %lld", (long long)h,
+                (long long)out_capacity);
+  return PJ_OK;
+}
+
+pj_status pj_project_reference(const pj_table* build, const pj_pair* pairs, int64_t n, void* out_xy,
+                               hipStream_t stream) {
+  pj_status st = check_table(build, "build");
+  if (st != PJ_OK) return st;
+  if (n < 0 || (n > 0 && (!pairs || !out_xy))) return fail(PJ_ERR_INVALID_ARG, "bad pairs/out_xy/n");
+  if (n == 0) return PJ_OK;
+  PJ_HIP(project_reference(build->keys, build->key_bytes, pairs, n, out_xy, stream));
+  return PJ_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Phase-level entry points
+// ---------------------------------------------------------------------------
+static void one_table_layout(Carver& c, const pj_table* t, const PartParams& pp, Layout* L) {
+  std::memset(L, 0, sizeof(*L));
+  L->ntables = 1;
+  L->pp = pp;
+  PartTable& T = L->t[0];
+  T.keys = t->keys;
+  T.pay = t->payload;
+  T.n = uint32_t(t->n);
+  T.nseg = choose_nseg(t->n);
+  T.mat = static_cast<uint32_t*>(c.take(size_t(T.nseg) * pp.P * 4));
+  T.matc = pp.passes == 2 ? static_cast<uint32_t*>(c.take(size_t(T.nseg) * pp.F1 * 4)) : nullptr;
+  L->t[1] = T;  // unused second slot mirrors the first
+}
+
+size_t pj_histogram_workspace_size(int32_t total_bits) {
+  if (total_bits < 1 || total_bits > kMaxTotalBits) return 0;
+  const size_t P = size_t(1) << total_bits;
+  return size_t(kMaxSegPerXcd * kNumXcd) * (P + 4096) * 4 + 4 * 256 + P * 4;
+}
+
+pj_status pj_combined_histogram(const pj_table* table, int32_t total_bits, int32_t bits2, uint32_t* d_fine,
+                                uint32_t* d_pass1, void* workspace, size_t workspace_bytes, hipStream_t stream) {
+  pj_status st = check_table(table, "input");
+  if (st != PJ_OK) return st;
+  if (total_bits < 1 || total_bits > kMaxTotalBits) return fail(PJ_ERR_INVALID_ARG, "total_bits must be in [1, 16]");
+  if (bits2 < 0 || total_bits - bits2 < 1 || total_bits - bits2 > 12)
+    return fail(PJ_ERR_INVALID_ARG, "pass-1 bits (total_bits - bits2) must be in [1, 12]");
+  if (!d_fine || !d_pass1) return fail(PJ_ERR_INVALID_ARG, "output histogram pointer is NULL");
+  if (workspace_bytes < pj_histogram_workspace_size(total_bits) || !workspace)
+    return fail(PJ_ERR_WORKSPACE, "histogram workspace too small");
+  PartParams pp{};
+  pp.B = total_bits;
+  pp.b2 = bits2;
+  pp.b1 = total_bits - bits2;
+  pp.P = 1 << total_bits;
+  pp.F1 = 1 << pp.b1;
+  pp.F2 = 1 << bits2;
+  pp.passes = bits2 > 0 ? 2 : 1;
+  Carver c{static_cast<char*>(workspace)};
+  Layout L;
+  one_table_layout(c, table, pp, &L);
+  L.t[0].tot = d_fine;
+  L.t[0].totc = d_pass1;
+  L.t[1] = L.t[0];
+  if (table->n == 0) {
+    PJ_HIP(launch_fill_u32(d_fine, 0, pp.P, stream));
+    PJ_HIP(launch_fill_u32(d_pass1, 0, pp.F1, stream));
+    return PJ_OK;
+  }
+  PJ_HIP(launch_hist(L, table->key_bytes, stream));
+  PJ_HIP(launch_colscan(L, stream));
+  if (pp.passes == 1) PJ_HIP(hipMemcpyAsync(d_pass1, d_fine, size_t(pp.P) * 4, hipMemcpyDeviceToDevice, stream));
+  return PJ_OK;
+}
+
+pj_status pj_histogram(const pj_table* table, int32_t shift, int32_t bits, uint32_t* d_hist, void* workspace,
+                       size_t workspace_bytes, hipStream_t stream) {
+  (void)workspace;
+  (void)workspace_bytes;
+  pj_status st = check_table(table, "input");
+  if (st != PJ_OK) return st;
+  if (bits < 1 || bits > 15) return fail(PJ_ERR_INVALID_ARG, "bits must be in [1, 15]");
+  if (shift < 0 || shift + bits > 8 * table->key_bytes) return fail(PJ_ERR_INVALID_ARG, "bad shift");
+  if (shift + bits > 32) return fail(PJ_ERR_UNSUPPORTED, "digits above bit 31 are not supported");
+  if (!d_hist) return fail(PJ_ERR_INVALID_ARG, "d_hist is NULL");
+  PJ_HIP(digit_hist(table->keys, table->n, table->key_bytes, 1, shift, bits, d_hist, stream));
+  return PJ_OK;
+}
+
+size_t pj_scan_workspace_size(int64_t n) { return scan_ws_bytes(n); }
+
+static pj_status scan_common(const uint32_t* in, uint32_t* out, int64_t n, void* ws, size_t wsb, hipStream_t s,
+                             int inclusive) {
+  if (n < 0 || (n > 0 && (!in || !out))) return fail(PJ_ERR_INVALID_ARG, "bad scan arguments");
+  if (n > (int64_t(1) << 34)) return fail(PJ_ERR_UNSUPPORTED, "scan length too large");
+  if (n == 0) return PJ_OK;
+  if (!ws || wsb < scan_ws_bytes(n)) return fail(PJ_ERR_WORKSPACE, "scan workspace too small");
+  PJ_HIP(scan_u32(in, out, n, inclusive, static_cast<uint32_t*>(ws), s));
+  return PJ_OK;
+}
+
+pj_status pj_exclusive_scan(const uint32_t* d_in, uint32_t* d_out, int64_t n, void* workspace, size_t workspace_bytes,
+                            hipStream_t stream) {
+  return scan_common(d_in, d_out, n, workspace, workspace_bytes, stream, 0);
+}
+
+pj_status pj_inclusive_scan(const uint32_t* d_in, uint32_t* d_out, int64_t n, void* workspace, size_t workspace_bytes,
+                            hipStream_t stream) {
+  return scan_common(d_in, d_out, n, workspace, workspace_bytes, stream, 1);
+}
+
+size_t pj_partition_workspace_size(const pj_table* table, int32_t bits) {
+  if (!table || bits < 1 || bits > 12 || table->n < 0) return 0;
+  const size_t P = size_t(1) << bits;
+  return size_t(choose_nseg(table->n)) * P * 4 + P * 4 + 1024;
+}
+
+pj_status pj_partition_pass(const pj_table* table, int32_t shift, int32_t bits, void* d_tuples_out,
+                            uint32_t* d_offsets, void* workspace, size_t workspace_bytes, hipStream_t stream) {
+  pj_status st = check_table(table, "input");
+  if (st != PJ_OK) return st;
+  if (bits < 1 || bits > 12) return fail(PJ_ERR_INVALID_ARG, "bits must be in [1, 12]");
+  if (shift < 0 || shift + bits > 32) return fail(PJ_ERR_INVALID_ARG, "shift + bits must be <= 32");
+  if (!d_offsets || (table->n > 0 && !d_tuples_out)) return fail(PJ_ERR_INVALID_ARG, "NULL output");
+  if (!workspace || workspace_bytes < pj_partition_workspace_size(table, bits))
+    return fail(PJ_ERR_WORKSPACE, "partition workspace too small");
+  PartParams pp{};
+  pp.B = bits;
+  pp.b1 = bits;
+  pp.b2 = 0;
+  pp.P = 1 << bits;
+  pp.F1 = pp.P;
+  pp.F2 = 1;
+  pp.passes = 1;
+  pp.shift0 = shift;
+  Carver c{static_cast<char*>(workspace)};
+  Layout L;
+  one_table_layout(c, table, pp, &L);
+  L.t[0].tot = static_cast<uint32_t*>(c.take(size_t(pp.P) * 4));
+  L.t[0].off = d_offsets;
+  L.t[0].buf1 = d_tuples_out;
+  L.t[1] = L.t[0];
+  PJ_HIP(launch_hist(L, table->key_bytes, stream));
+  PJ_HIP(launch_colscan(L, stream));
+  PJ_HIP(launch_offsets(L, stream));
+  if (table->n > 0) PJ_HIP(launch_pass1(L, table->key_bytes, stream));
+  return PJ_OK;
+}
+
+size_t pj_partition2_workspace_size(int64_t n, int32_t bits1, int32_t bits2) {
+  (void)n;
+  if (bits1 < 1 || bits1 > 12 || bits2 < 1 || bits2 > 12 || bits1 + bits2 > kMaxTotalBits) return 0;
+  const size_t P = size_t(1) << (bits1 + bits2), F1 = size_t(1) << bits1;
+  return size_t(kNumXcd) * (P + F1) * 4 + P * 4 + 4 * 256;
+}
+
+pj_status pj_partition_pass2(const void* d_tuples_in, int64_t n, int32_t key_bytes, int32_t bits1, int32_t bits2,
+                             const uint32_t* d_offsets1, void* d_tuples_out, uint32_t* d_offsets2, void* workspace,
+                             size_t workspace_bytes, hipStream_t stream) {
+  if (key_bytes != 4 && key_bytes != 8) return fail(PJ_ERR_INVALID_ARG, "key_bytes must be 4 or 8");
+  if (n < 0 || n >= (int64_t(1) << 31)) return fail(PJ_ERR_INVALID_ARG, "n outside [0, 2^31)");
+  if (bits1 < 1 || bits1 > 12 || bits2 < 1 || bits2 > 12 || bits1 + bits2 > kMaxTotalBits)
+    return fail(PJ_ERR_INVALID_ARG, "bits1, bits2 must be in [1, 12] with bits1 + bits2 <= 16");
+  if (!d_offsets1 || !d_offsets2 || (n > 0 && (!d_tuples_in || !d_tuples_out)))
+    return fail(PJ_ERR_INVALID_ARG, "NULL argument");
+  if (!workspace || workspace_bytes < pj_partition2_workspace_size(n, bits1, bits2))
+    return fail(PJ_ERR_WORKSPACE, "pass-2 workspace too small");
+  PartParams pp{};
+  pp.b1 = bits1;
+  pp.b2 = bits2;
+  pp.B = bits1 + bits2;
+  pp.P = 1 << pp.B;
+  pp.F1 = 1 << bits1;
+  pp.F2 = 1 << bits2;
+  pp.passes = 2;
+  Carver c{static_cast<char*>(workspace)};
+  Layout L{};
+  L.ntables = 1;
+  L.pp = pp;
+  PartTable& T = L.t[0];
+  T.n = uint32_t(n);
+  T.nseg = kNumXcd;
+  T.buf1 = const_cast<void*>(d_tuples_in);
+  T.buf2 = d_tuples_out;
+  T.mat = static_cast<uint32_t*>(c.take(size_t(kNumXcd) * pp.P * 4));
+  T.matc = static_cast<uint32_t*>(c.take(size_t(kNumXcd) * pp.F1 * 4));
+  T.tot = static_cast<uint32_t*>(c.take(size_t(pp.P) * 4));
+  T.off = d_offsets2;
+  L.t[1] = T;
+  PJ_HIP(digit_hist(d_tuples_in, n, key_bytes, 2, 0, pp.B, T.tot, stream));
+  PJ_HIP(launch_offsets(L, stream));
+  PJ_HIP(p2_prep(d_offsets1, pp.F1, T.nseg, T.matc, T.mat, pp.P, stream));
+  if (n > 0) PJ_HIP(launch_pass2(L, key_bytes, stream));
+  return PJ_OK;
+}
+
+size_t pj_assign_blocks_workspace_size(int32_t num_partitions) {
+  return num_partitions < 1 ? 0 : scan_ws_bytes(num_partitions);
+}
+
+pj_status pj_assign_blocks(const uint32_t* d_offsets, int64_t total, int32_t num_partitions, int32_t base,
+                           uint32_t* d_blocks_scanned, uint32_t* d_partition_index, int64_t max_blocks,
+                           uint32_t* d_total_blocks, void* workspace, size_t workspace_bytes, hipStream_t stream) {
+  if (num_partitions < 1 || base < 1 || total < 0 || total >= (int64_t(1) << 31) || max_blocks < 0)
+    return fail(PJ_ERR_INVALID_ARG, "bad assign_blocks sizes");
+  if (!d_offsets || !d_blocks_scanned || !d_total_blocks || (max_blocks > 0 && !d_partition_index))
+    return fail(PJ_ERR_INVALID_ARG, "NULL argument");
+  if (!workspace || workspace_bytes < pj_assign_blocks_workspace_size(num_partitions))
+    return fail(PJ_ERR_WORKSPACE, "assign_blocks workspace too small");
+  PJ_HIP(assign_blocks(d_offsets, total, num_partitions, base, d_blocks_scanned, d_partition_index, max_blocks,
+                       d_total_blocks, static_cast<uint32_t*>(workspace), stream));
+  return PJ_OK;
+}
+
+size_t pj_join_workspace_size(int32_t total_bits) {
+  if (total_bits < 1 || total_bits > kMaxTotalBits) return 0;
+  return join_generic_ws_bytes(total_bits) + 4096;
+}
+
+pj_status pj_join_partitions(const void* d_build_tuples, const uint32_t* d_build_offsets, const void* d_probe_tuples,
+                             const uint32_t* d_probe_offsets, int32_t key_bytes, int32_t total_bits, int32_t base,
+                             int64_t probe_slice, int32_t unique_build_keys, pj_pair* out, int64_t out_capacity,
+                             int64_t* d_out_count, void* workspace, size_t workspace_bytes, hipStream_t stream) {
+  if (key_bytes != 4 && key_bytes != 8) return fail(PJ_ERR_INVALID_ARG, "key_bytes must be 4 or 8");
+  if (total_bits < 1 || total_bits > kMaxTotalBits) return fail(PJ_ERR_INVALID_ARG, "total_bits must be in [1, 16]");
+  if (!d_build_offsets || !d_probe_offsets || !d_out_count) return fail(PJ_ERR_INVALID_ARG, "NULL argument");
+  if (out_capacity < 0 || (out_capacity > 0 && !out)) return fail(PJ_ERR_INVALID_ARG, "bad output");
+  if (!workspace || workspace_bytes < pj_join_workspace_size(total_bits))
+    return fail(PJ_ERR_WORKSPACE, "join workspace too small");
+  if (reinterpret_cast<uintptr_t>(workspace) % 256 != 0)
+    return fail(PJ_ERR_INVALID_ARG, "workspace must be 256-byte aligned");
+  const int cap = join_table_capacity(key_bytes) / 2 - 1;
+  if (base < 1) base = 1;
+  if (base > cap) base = cap;
+  if (probe_slice <= 0) probe_slice = int64_t(base) * 8 < 16384 ? 16384 : int64_t(base) * 8;
+  PJ_HIP(launch_join_generic(d_build_tuples, d_build_offsets, d_probe_tuples, d_probe_offsets, key_bytes, total_bits,
+                             base, probe_slice, unique_build_keys, out, out_capacity, d_out_count,
+                             static_cast<char*>(workspace), stream));
+  return PJ_OK;
+}
+
+pj_status pj_hash_keys(const void* d_keys, int64_t n, int32_t key_bytes, uint32_t* d_out, hipStream_t stream) {
+  if ((key_bytes != 4 && key_bytes != 8) || n < 0 || (n > 0 && (!d_keys || !d_out)))
+    return fail(PJ_ERR_INVALID_ARG, "bad hash_keys arguments");
+  if (n == 0) return PJ_OK;
+  PJ_HIP(hash_keys(d_keys, n, key_bytes, d_out, stream));
+  return PJ_OK;
+}
+
+pj_status pj_fastdiv(const int32_t* d_n, int64_t count, int32_t divisor, int32_t* d_q, int32_t* d_r,
+                     hipStream_t stream) {
+  if (divisor == 0) return fail(PJ_ERR_INVALID_ARG, "divisor must be non-zero");
+  if (count < 0 || (count > 0 && !d_n)) return fail(PJ_ERR_INVALID_ARG, "bad fastdiv arguments");
+  if (count == 0) return PJ_OK;
+  PJ_HIP(fastdiv_run(d_n, count, divisor, d_q, d_r, stream));
+  return PJ_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Generators
+// ---------------------------------------------------------------------------
+#define PJ_GEN_CHECK(ptr, n)                                                              \
+  do {                                                                                    \
+    if ((n) < 0 || ((n) > 0 && !(ptr))) return fail(PJ_ERR_INVALID_ARG, "bad generator output"); \
+    if ((n) == 0) return PJ_OK;                                                           \
+  } while (0)
+
+pj_status pj_gen_build_unique_i32(int32_t* d_keys, int64_t n, int64_t domain, uint64_t perm_seed,
+                                  hipStream_t stream) {
+  PJ_GEN_CHECK(d_keys, n);
+  if (domain < n || domain > (int64_t(1) << 32)) return fail(PJ_ERR_INVALID_ARG, "domain must be in [n, 2^32]");
+  PJ_HIP(gen_build_unique_i32(d_keys, n, domain, perm_seed, stream));
+  return PJ_OK;
+}
+
+pj_status pj_gen_build_dup_i32(int32_t* d_keys, int64_t n, int64_t m, int64_t domain, uint64_t perm_seed,
+                               uint64_t seed, hipStream_t stream) {
+  PJ_GEN_CHECK(d_keys, n);
+  if (m < 1 || m > domain || domain > (int64_t(1) << 32)) return fail(PJ_ERR_INVALID_ARG, "need 1 <= m <= domain");
+  PJ_HIP(gen_build_dup_i32(d_keys, n, m, domain, perm_seed, seed, stream));
+  return PJ_OK;
+}
+
+pj_status pj_gen_probe_i32(int32_t* d_keys, int64_t n, int64_t domain, int64_t n_build, uint64_t perm_seed,
+                           uint64_t seed, double sel, int64_t nonmatch_base, int64_t nonmatch_range,
+                           hipStream_t stream) {
+  PJ_GEN_CHECK(d_keys, n);
+  if (n_build < 1 || n_build > domain || domain > (int64_t(1) << 32) || nonmatch_range < 0)
+    return fail(PJ_ERR_INVALID_ARG, "bad probe generator sizes");
+  if (nonmatch_range == 0 && domain == n_build && sel < 1.0)
+    return fail(PJ_ERR_INVALID_ARG, "no non-matching keys left in the domain");
+  PJ_HIP(gen_probe_i32(d_keys, n, domain, n_build, perm_seed, seed, sel, nonmatch_base, nonmatch_range, stream));
+  return PJ_OK;
+}
+
+pj_status pj_zipf_lut(double* h_lut, int64_t n, double theta) {
+  if (!h_lut || n < 1) return fail(PJ_ERR_INVALID_ARG, "bad zipf table arguments");
+  double scale = 0.0;
+  for (int64_t i = 1; i <= n; ++i) scale += 1.0 / std::pow(double(i), theta);
+  double sum = 0.0;
+  for (int64_t i = 1; i <= n; ++i) {
+    sum += 1.0 / std::pow(double(i), theta);
+    h_lut[i - 1] = sum / scale;
+  }
+  return PJ_OK;
+}
+
+pj_status pj_gen_probe_zipf_i32(int32_t* d_keys, int64_t count, const double* d_lut, int64_t n,
+                                uint64_t rank_perm_seed, uint64_t seed, hipStream_t stream) {
+  PJ_GEN_CHECK(d_keys, count);
+  if (!d_lut || n < 1 || n > (int64_t(1) << 32)) return fail(PJ_ERR_INVALID_ARG, "bad zipf arguments");
+  PJ_HIP(gen_probe_zipf_i32(d_keys, count, d_lut, n, rank_perm_seed, seed, stream));
+  return PJ_OK;
+}
+
+pj_status pj_gen_build_i64(int64_t* d_keys, int64_t n, uint64_t key_seed, hipStream_t stream) {
+  PJ_GEN_CHECK(d_keys, n);
+  PJ_HIP(gen_build_i64(d_keys, n, key_seed, stream));
+  return PJ_OK;
+}
+
+pj_status pj_gen_probe_i64(int64_t* d_keys, int64_t n, int64_t n_build, uint64_t key_seed, uint64_t seed, double sel,
+                           hipStream_t stream) {
+  PJ_GEN_CHECK(d_keys, n);
+  if (n_build < 1) return fail(PJ_ERR_INVALID_ARG, "n_build must be >= 1");
+  PJ_HIP(gen_probe_i64(d_keys, n, n_build, key_seed, seed, sel, stream));
+  return PJ_OK;
+}
+
+pj_status pj_gen_probe_zipf_i64(int64_t* d_keys, int64_t count, const double* d_lut, int64_t n_build,
+                                uint64_t key_seed, uint64_t rank_perm_seed, uint64_t seed, hipStream_t stream) {
+  PJ_GEN_CHECK(d_keys, count);
+  if (!d_lut || n_build < 1) return fail(PJ_ERR_INVALID_ARG, "bad zipf arguments");
+  PJ_HIP(gen_probe_zipf_i64(d_keys, count, d_lut, n_build, key_seed, rank_perm_seed, seed, stream));
+  return PJ_OK;
+}
+
+pj_status pj_gen_row_ids(int32_t* d_pay, int64_t n, hipStream_t stream) {
+  PJ_GEN_CHECK(d_pay, n);
+  PJ_HIP(gen_row_ids(d_pay, n, stream));
+  return PJ_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Profiling
+// ---------------------------------------------------------------------------
+pj_status pj_profile_enable(int32_t on) {
+  g_prof.on = on != 0;
+  g_prof.recorded = false;
+  return PJ_OK;
+}
+
+pj_status pj_profile_read(pj_phase_times* t) {
+  if (!t) return fail(PJ_ERR_INVALID_ARG, "times is NULL");
+  std::memset(t, 0, sizeof(*t));
+  if (!g_prof.on || !g_prof.recorded) return PJ_OK;
+  PJ_HIP(hipEventSynchronize(g_prof.ev[7]));
+  float ms[7];
+  for (int i = 0; i < 7; ++i) PJ_HIP(hipEventElapsedTime(&ms[i], g_prof.ev[i], g_prof.ev[i + 1]));
+  t->histogram_ms = ms[0];
+  t->offsets_ms = ms[1];
+  t->pass1_ms = ms[2];
+  t->pass2_ms = ms[3];
+  t->skew_ms = ms[4];
+  t->join_ms = ms[5];
+  t->compact_ms = ms[6];
+  PJ_HIP(hipEventElapsedTime(&t->total_ms, g_prof.ev[0], g_prof.ev[7]));
+  t->valid = 1;
+  return PJ_OK;
+}
+
+}  // extern "C"

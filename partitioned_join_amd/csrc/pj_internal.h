@@ -1,0 +1,123 @@
+// pj_internal.h -- structures shared by the kernel files and the host orchestration.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "../../include/pj.h"
+
+namespace pj {
+
+// Tuning constants (gfx950: 256 CUs, 160 KB LDS per CU, wave64).
+constexpr int kHistBlock = 1024;      // combined histogram workgroup
+constexpr int kScatterBlock = 512;    // partition pass workgroup
+constexpr int kScatterTile4 = 8192;   // tuples staged in LDS per tile, 4-byte keys (64 KB)
+constexpr int kScatterTile8 = 4096;   // 8-byte keys (64 KB)
+constexpr int kJoinBlock = 512;       // join workgroup
+constexpr int kJoinStage = 2048;      // output pairs staged in LDS before a flush
+constexpr int64_t kOutChunk = 16384;  // output pairs claimed per global allocation
+constexpr int kJoinGrid = 512;        // persistent join workgroups (2 per CU)
+constexpr int kPass2WgPerXcd = 32;    // pass-2 persistent workgroups per XCD and table
+constexpr int kPass2Groups = 16;      // segment groups per pass-1 partition in pass 2
+constexpr int kMaxTotalBits = 16;     // combined histogram limit (65536 partitions)
+constexpr int kMaxSegPerXcd = 64;
+
+// One table being partitioned.  All pointers are device pointers.
+struct PartTable {
+  const void* keys;      // SoA keys (pass-1 input)
+  const int32_t* pay;    // SoA payload or nullptr (row ids)
+  uint32_t n;            // rows
+  uint32_t nseg;         // histogram / pass-1 segments (multiple of 8)
+  void* buf1;            // pass-1 output tuples
+  void* buf2;            // pass-2 output tuples (2-pass plans)
+  uint32_t* mat;         // [nseg][P] per-segment fine counts; exclusive column scan after colscan
+  uint32_t* matc;        // [nseg][F1] per-segment pass-1 counts (2-pass); scanned likewise
+  uint32_t* tot;         // [P] fine totals
+  uint32_t* totc;        // [F1] pass-1 totals (2-pass)
+  uint32_t* off;         // [P+1] final partition starts, off[P] = n
+};
+
+struct PartParams {
+  int B;        // total radix bits
+  int b1, b2;   // pass bits (b2 == 0 for one pass)
+  int P, F1, F2;
+  int passes;
+  int shift0;   // every digit is taken from key >> shift0 (0 in the join pipeline)
+};
+
+// Segment s of nseg splits [0, n) at multiples of 64 rows.
+__host__ __device__ inline uint32_t seg_bound(uint32_t n, uint32_t nseg, uint32_t k) {
+  if (k == 0) return 0;
+  if (k >= nseg) return n;
+  return uint32_t((uint64_t(n) * k / nseg) & ~uint64_t(63));
+}
+
+struct JoinArgs {
+  const void* rt;          // build tuples (partitioned)
+  const uint32_t* roff;    // [P+1]
+  const void* st;          // probe tuples (partitioned)
+  const uint32_t* soff;    // [P+1]
+  int P;
+  int base;                // build chunk per item
+  int64_t slice;           // probe slice per item
+  const uint64_t* item_scan;  // [P+1]
+  pj_pair* out;
+  int64_t capacity;
+  pj_pair* spill;          // kJoinGrid * kOutChunk pairs
+  uint32_t* chunk_ctr;     // claimed output chunks
+  uint64_t* wg_last;       // [grid] (chunk << 32 | fill) or ~0 when no chunk; fill <= kOutChunk
+  int table_cap;           // LDS slots available for the build table
+  uint32_t nr, ns;         // table sizes (bounds for the item ranges)
+};
+
+struct DefragState {
+  uint64_t total;          // pairs found
+  uint64_t moves;          // pairs to move
+  uint32_t n_holes, n_strays;
+  uint32_t pad[2];
+};
+
+// Workspace carve-up for one join.
+struct Layout {
+  PartTable t[2];
+  int ntables;         // 1 or 2 (phase API runs one table)
+  PartParams pp;
+  uint64_t* item_scan;
+  uint32_t* chunk_ctr;
+  uint64_t* wg_last;
+  pj_pair* spill;
+  DefragState* dstate;
+  uint64_t* hole_lo;   // [kJoinGrid + 2]
+  uint64_t* hole_pre;  // [kJoinGrid + 3]
+  uint64_t* stray_lo;  // [kJoinGrid + 2]
+  uint64_t* stray_pre; // [kJoinGrid + 3]
+  size_t bytes;
+};
+
+int tuple_bytes(int key_bytes);
+uint32_t choose_nseg(int64_t n);
+size_t layout_join(int64_t nR, int64_t nS, int key_bytes, const PartParams& pp, char* base, Layout* L);
+
+// Launchers (return hipError_t of the launch).
+hipError_t launch_fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t s);
+hipError_t launch_fill_u64(uint64_t* p, uint64_t v, int64_t n, hipStream_t s);
+hipError_t launch_hist(const Layout& L, int key_bytes, hipStream_t s);
+hipError_t launch_colscan(const Layout& L, hipStream_t s);
+hipError_t launch_offsets(const Layout& L, hipStream_t s);
+hipError_t launch_pass1(const Layout& L, int key_bytes, hipStream_t s);
+hipError_t launch_pass2(const Layout& L, int key_bytes, hipStream_t s);
+hipError_t launch_join_plan(const Layout& L, int base, int64_t slice, hipStream_t s);
+hipError_t launch_join(const Layout& L, int key_bytes, int base, int64_t slice, int unique, pj_pair* out,
+                       int64_t capacity, hipStream_t s);
+hipError_t launch_defrag(const Layout& L, pj_pair* out, int64_t capacity, int64_t* d_count, hipStream_t s);
+
+// Generic join over externally partitioned tuples (phase API).
+hipError_t launch_join_generic(const void* rt, const uint32_t* roff, const void* st, const uint32_t* soff,
+                               int key_bytes, int B, int base, int64_t slice, int unique, pj_pair* out,
+                               int64_t capacity, int64_t* d_count, char* ws, hipStream_t s);
+size_t join_generic_ws_bytes(int B);
+
+int join_table_capacity(int key_bytes);
+
+}  // namespace pj

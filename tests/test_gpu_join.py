@@ -1,0 +1,190 @@
+"""GPU parity of the full join (pj_partitioned_join through the C ABI) against
+the oracle's CPU join: sorted (build_idx, probe_idx) multisets must be equal.
+Covers the benchmark configurations at oracle-checkable sizes, the reference's
+own edge cases and, at full size, size-independent properties."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch as t
+    assert t.cuda.is_available(), "GPU tests need a HIP device"
+    return t
+
+
+def sorted_np(p):
+    p = np.asarray(p, dtype=np.int64).reshape(-1, 2)
+    return p[np.lexsort((p[:, 1], p[:, 0]))]
+
+
+def check_join(pj, oracle, torch, bk, pk, bp=None, pp=None, plan=None, uniq=True):
+    d = lambda a: None if a is None else torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    got = pj.partitioned_join(d(bk), d(pk), d(bp), d(pp), plan=plan)
+    exp = oracle.inner_join_cpu(bk, pk, bp, pp, uniq_keys=uniq)
+    g = oracle.sort_pairs(got.cpu().numpy())
+    e = oracle.sort_pairs(exp)
+    assert len(g) == len(e), (len(g), len(e))
+    assert np.array_equal(g, e)
+    return g
+
+
+def test_known_answers(pj, oracle, torch):
+    with open(os.path.join(GOLDEN, "cudf_kat.json")) as f:
+        cases = json.load(f)
+    for c in cases:
+        bk = np.array(c["build"], dtype=np.int32)
+        pk = np.array(c["probe"], dtype=np.int32)
+        d = lambda a: torch.from_numpy(a).cuda()
+        got = pj.partitioned_join(d(bk), d(pk))
+        assert np.array_equal(sorted_np(got.cpu().numpy()), sorted_np(c["pairs"])), c["name"]
+
+
+def test_reference_join_vectors_projection(pj, oracle, torch):
+    # the reference's own inner_join_cpu output {key, 0}, compared through the projection
+    with open(os.path.join(GOLDEN, "ref_join.json")) as f:
+        cases = json.load(f)
+    for c in cases:
+        bk = torch.tensor(c["build"], dtype=torch.int32, device="cuda")
+        pk = torch.tensor(c["probe"], dtype=torch.int32, device="cuda")
+        pairs = pj.partitioned_join(bk, pk, plan={"passes": 1, "bits1": 4, "unique_build_keys": c["uniq"]})
+        assert pairs.shape[0] == c["count"]
+        xy = pj.project_reference(bk, pairs).cpu().numpy()
+        xy[:, 1] = 0
+        assert np.array_equal(sorted_np(xy), sorted_np(c["xy"]))
+
+
+def test_config1_exact(pj, oracle, torch):
+    """1M unique build keys (perm of [0, 2^20)) x 4M probes drawn from them:
+    exactly 4,194,304 pairs, every probe row exactly once."""
+    bk, pk = pj.generators.workload_config1()
+    pairs = pj.partitioned_join(bk, pk)
+    assert pairs.shape[0] == 1 << 22
+    b, p = bk.cpu().numpy(), pk.cpu().numpy()
+    exp = oracle.inner_join_cpu(b, p)
+    assert np.array_equal(oracle.sort_pairs(pairs.cpu().numpy()), oracle.sort_pairs(exp))
+    assert np.array_equal(np.sort(pairs[:, 1].cpu().numpy()), np.arange(1 << 22))
+
+
+@pytest.mark.parametrize("plan", [None, {"passes": 1, "bits1": 6}, {"passes": 2, "bits1": 5, "bits2": 7},
+                                  {"passes": 2, "bits1": 8, "bits2": 8, "unique_build_keys": 1},
+                                  {"passes": 1, "bits1": 12, "max_partition_size_factor": 1.0},
+                                  {"passes": 2, "bits1": 3, "bits2": 3, "probe_slice": 1000}])
+def test_config2_shape_scaled(pj, oracle, torch, plan):
+    """Config-2 distribution at 1/16 size: 4M unique keys x 16M probes, 50 % match."""
+    bk, pk = pj.generators.workload_config2(scale_shift=4)
+    check_join(pj, oracle, torch, bk.cpu().numpy(), pk.cpu().numpy(), plan=plan)
+
+
+@pytest.mark.parametrize("theta", [0.5, 1.0, 1.5])
+def test_config3_zipf_scaled(pj, oracle, torch, theta):
+    """Config-3 distribution at 1/16 size: 1M unique keys x 16M Zipf probes."""
+    bk, pk = pj.generators.workload_config3(theta, scale_shift=4)
+    g = check_join(pj, oracle, torch, bk.cpu().numpy(), pk.cpu().numpy())
+    assert len(g) == pk.numel()
+
+
+def test_duplicate_build_keys(pj, oracle, torch):
+    rng = np.random.default_rng(21)
+    bk = rng.integers(0, 5000, 200000).astype(np.int32)
+    pk = rng.integers(0, 6000, 300000).astype(np.int32)
+    check_join(pj, oracle, torch, bk, pk, uniq=False)
+    check_join(pj, oracle, torch, bk, pk, plan={"passes": 2, "bits1": 4, "bits2": 4}, uniq=False)
+
+
+def test_all_equal_keys_large_output(pj, oracle, torch):
+    # like cuDF's HashJoinLargeOutputSize, scaled: 3000 x 4000 identical keys -> 12M pairs
+    bk = np.zeros(3000, dtype=np.int32)
+    pk = np.zeros(4000, dtype=np.int32)
+    got = pj.partitioned_join(torch.from_numpy(bk).cuda(), torch.from_numpy(pk).cuda())
+    assert got.shape[0] == 12_000_000
+    g = got.cpu().numpy().astype(np.int64)
+    code = np.sort(g[:, 0] * 4000 + g[:, 1])
+    assert np.array_equal(code, np.arange(12_000_000))
+
+
+def test_payload_columns(pj, oracle, torch):
+    rng = np.random.default_rng(22)
+    bk = rng.permutation(1 << 18).astype(np.int32)
+    pk = rng.integers(0, 1 << 19, 1 << 20).astype(np.int32)
+    bp = rng.integers(-2**31, 2**31 - 1, len(bk)).astype(np.int32)
+    pp = rng.integers(-2**31, 2**31 - 1, len(pk)).astype(np.int32)
+    check_join(pj, oracle, torch, bk, pk, bp, pp)
+    check_join(pj, oracle, torch, bk, pk, bp, None)
+
+
+def test_edge_keys_and_sizes(pj, oracle, torch):
+    extremes = np.array([-1, 0, 1, -2**31, 2**31 - 1, -2, 65535, 65536, -65536], dtype=np.int32)
+    bk = np.concatenate([extremes, np.arange(1000, dtype=np.int32) * 65536])
+    pk = np.concatenate([extremes, extremes, np.arange(2000, dtype=np.int32) * 32768])
+    for plan in [None, {"passes": 1, "bits1": 1}, {"passes": 2, "bits1": 8, "bits2": 8}]:
+        check_join(pj, oracle, torch, bk, pk, plan=plan)
+    one = np.array([7], dtype=np.int32)
+    check_join(pj, oracle, torch, one, one)
+    check_join(pj, oracle, torch, one, np.array([8], dtype=np.int32))
+
+
+def test_empty_tables(pj, torch):
+    e = torch.empty(0, dtype=torch.int32, device="cuda")
+    x = torch.arange(10, dtype=torch.int32, device="cuda")
+    assert pj.partitioned_join(e, x).shape[0] == 0
+    assert pj.partitioned_join(x, e).shape[0] == 0
+    assert pj.partitioned_join(e, e).shape[0] == 0
+
+
+def test_int64_keys(pj, oracle, torch):
+    G = pj.generators
+    bk = G.build_i64(300000, 0x64)
+    pk = G.probe_i64(900000, 300000, 0x64, 0x65, 0.6)
+    check_join(pj, oracle, torch, bk.cpu().numpy(), pk.cpu().numpy())
+    lut = oracle.zipf_lut(300000, 1.2)
+    pz = G.probe_zipf_i64(900000, 300000, 1.2, 0x64, 0x66, 0x67, lut=lut)
+    check_join(pj, oracle, torch, bk.cpu().numpy(), pz.cpu().numpy(), plan={"passes": 2, "bits1": 4, "bits2": 4})
+    rng = np.random.default_rng(23)
+    dup = rng.integers(-3, 3, 2000).astype(np.int64) * (2**40 + 1)
+    check_join(pj, oracle, torch, dup, dup[::-1].copy(), uniq=False)
+
+
+def test_capacity_error_reports_count(pj, torch):
+    from partitioned_join_amd import PjError
+    bk = torch.arange(1000, dtype=torch.int32, device="cuda")
+    pk = torch.arange(5000, dtype=torch.int32, device="cuda") % 1000
+    with pytest.raises(PjError) as ei:
+        pj.partitioned_join(bk, pk, out_capacity=100)
+    assert ei.value.status == 2 and ei.value.count == 5000
+    ok = pj.partitioned_join(bk, pk, out_capacity=5000)
+    assert ok.shape[0] == 5000
+
+
+def test_repeat_calls_reuse_workspace(pj, oracle, torch):
+    bk, pk = pj.generators.workload_config2(scale_shift=6)
+    a = oracle.sort_pairs(pj.partitioned_join(bk, pk).cpu().numpy())
+    b = oracle.sort_pairs(pj.partitioned_join(bk, pk).cpu().numpy())
+    small_b, small_p = bk[:1000].clone(), pk[:5000].clone()
+    c = oracle.sort_pairs(pj.partitioned_join(small_b, small_p).cpu().numpy())
+    assert np.array_equal(a, b)
+    assert np.array_equal(c, oracle.sort_pairs(oracle.inner_join_cpu(small_b.cpu().numpy(), small_p.cpu().numpy())))
+
+
+def test_config2_full_size_properties(pj, torch):
+    """64M x 256M at full size, checked on the device through properties that
+    do not need the CPU oracle: the pair count equals the number of probe keys
+    below 2^26, every such probe row appears exactly once, and every pair joins
+    equal keys."""
+    bk, pk = pj.generators.workload_config2()
+    pairs, n = pj.partitioned_join(bk, pk, return_count=True)
+    match = pk < (1 << 26)
+    assert n == int(match.sum().item())
+    b = pairs[:, 0].long()
+    p = pairs[:, 1].long()
+    assert bool((bk[b] == pk[p]).all().item())
+    seen = torch.zeros(pk.numel(), dtype=torch.int32, device="cuda")
+    seen.index_add_(0, p, torch.ones_like(pairs[:, 1]))
+    assert bool((seen == match.int()).all().item())

@@ -1,0 +1,189 @@
+"""GPU parity of the device primitives and of each partitioning phase, called
+through the C ABI and checked against the oracle / golden vectors."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch as t
+    assert t.cuda.is_available(), "GPU tests need a HIP device"
+    return t
+
+
+def load(name):
+    with open(os.path.join(GOLDEN, name)) as f:
+        return json.load(f)
+
+
+def u32(t):
+    return t.cpu().numpy().view(np.uint32).astype(np.int64)
+
+
+def test_hash_keys_match_reference_vectors(pj, torch):
+    g = load("ref_murmur3.json")
+    k4 = torch.tensor(g["key4"], dtype=torch.int32, device="cuda")
+    assert u32(pj.hash_keys(k4)).tolist() == g["hash4"]
+    k8 = torch.tensor(g["key8"], dtype=torch.int64, device="cuda")
+    assert u32(pj.hash_keys(k8)).tolist() == g["hash8"]
+
+
+def test_fastdiv_matches_reference_vectors(pj, torch):
+    g = load("ref_fastdiv.json")
+    n = torch.tensor(g["n"], dtype=torch.int32, device="cuda")
+    for i, d in enumerate(g["d"]):
+        q, r = pj.fastdiv(n, d)
+        assert q.cpu().tolist() == g["q"][i], d
+        assert r.cpu().tolist() == g["r"][i], d
+
+
+def test_generators_bit_exact(pj, oracle, torch):
+    G = pj.generators
+    assert np.array_equal(G.build_unique_i32(70001, 90000, 0x1D).cpu().numpy(),
+                          oracle.gen_build_unique_i32(70001, 90000, 0x1D))
+    assert np.array_equal(G.build_dup_i32(5000, 300, 1000, 3, 4).cpu().numpy(),
+                          oracle.gen_build_dup_i32(5000, 300, 1000, 3, 4))
+    assert np.array_equal(G.probe_i32(50000, 4096, 4096, 7, 8, 0.5, 4096, 4096).cpu().numpy(),
+                          oracle.gen_probe_i32(50000, 4096, 4096, 7, 8, 0.5, 4096, 4096))
+    assert np.array_equal(G.probe_i32(50000, 9000, 4096, 7, 8, 0.3).cpu().numpy(),
+                          oracle.gen_probe_i32(50000, 9000, 4096, 7, 8, 0.3))
+    lut = oracle.zipf_lut(10000, 1.0)
+    assert np.array_equal(G.probe_zipf_i32(40000, 10000, 1.0, 5, 6).cpu().numpy(),
+                          oracle.gen_probe_zipf_i32(40000, lut, 5, 6))
+    assert np.array_equal(G.build_i64(3000, 11).cpu().numpy(), oracle.gen_build_i64(3000, 11))
+    assert np.array_equal(G.probe_i64(9000, 3000, 11, 12, 0.4).cpu().numpy(), oracle.gen_probe_i64(9000, 3000, 11, 12, 0.4))
+    assert np.array_equal(G.probe_zipf_i64(9000, 3000, 1.2, 11, 13, 14).cpu().numpy(),
+                          oracle.gen_probe_zipf_i64(9000, oracle.zipf_lut(3000, 1.2), 11, 13, 14))
+
+
+@pytest.mark.parametrize("n,total_bits,bits2", [(1, 4, 2), (1000, 8, 4), (300000, 10, 0), (2000000, 15, 7),
+                                                (3000000, 16, 8), (777777, 16, 4)])
+def test_combined_histogram(pj, oracle, torch, n, total_bits, bits2):
+    rng = np.random.default_rng(n)
+    keys = rng.integers(-2**31, 2**31 - 1, n).astype(np.int32)
+    d = torch.from_numpy(keys).cuda()
+    fine, p1 = pj.combined_histogram_pass(d, total_bits, bits2)
+    assert np.array_equal(u32(fine), oracle.histogram(keys, 0, total_bits))
+    assert np.array_equal(u32(p1), oracle.histogram(keys, bits2, total_bits - bits2))
+
+
+def test_combined_histogram_skewed_16bit_overflow_epochs(pj, oracle, torch):
+    # one bin receives far more than 65535 keys inside a single segment
+    n = 3 << 20
+    keys = np.full(n, 0x12345, dtype=np.int32)
+    keys[::7] = np.arange(len(keys[::7]), dtype=np.int32)
+    fine, p1 = pj.combined_histogram_pass(torch.from_numpy(keys).cuda(), 16, 8)
+    assert np.array_equal(u32(fine), oracle.histogram(keys, 0, 16))
+    assert np.array_equal(u32(p1), oracle.histogram(keys, 8, 8))
+
+
+def test_combined_histogram_int64(pj, oracle, torch):
+    rng = np.random.default_rng(5)
+    keys = rng.integers(-2**63, 2**63 - 1, 500000, dtype=np.int64)
+    fine, p1 = pj.combined_histogram_pass(torch.from_numpy(keys).cuda(), 12, 6)
+    assert np.array_equal(u32(fine), oracle.histogram(keys, 0, 12))
+    assert np.array_equal(u32(p1), oracle.histogram(keys, 6, 6))
+
+
+@pytest.mark.parametrize("shift,bits", [(0, 8), (8, 8), (5, 11), (20, 12)])
+def test_histogram_pass(pj, oracle, torch, shift, bits):
+    rng = np.random.default_rng(bits)
+    keys = rng.integers(-2**31, 2**31 - 1, 123457).astype(np.int32)
+    h = pj.histogram_pass(torch.from_numpy(keys).cuda(), shift, bits)
+    assert np.array_equal(u32(h), oracle.histogram(keys, shift, bits))
+
+
+@pytest.mark.parametrize("n", [1, 5, 1023, 16384, 16385, 1000003])
+def test_scans(pj, oracle, torch, n):
+    rng = np.random.default_rng(n)
+    x = rng.integers(0, 1000, n).astype(np.int32)
+    d = torch.from_numpy(x).cuda()
+    assert np.array_equal(u32(pj.exclusive_scan(d)), oracle.exclusive_scan(x) % 2**32)
+    assert np.array_equal(u32(pj.inclusive_scan(d)), oracle.inclusive_scan(x) % 2**32)
+
+
+def tuples_np(t, key_bytes):
+    a = t.cpu().numpy()
+    if key_bytes == 4:
+        return a[:, 0].copy(), a[:, 1].copy()
+    keys = a[:, :2].copy().view(np.int64).reshape(-1)
+    return keys, a[:, 2].copy()
+
+
+@pytest.mark.parametrize("n,shift,bits,with_pay", [(0, 0, 4, False), (1, 0, 1, False), (100, 0, 3, True),
+                                                   (65536 + 17, 0, 8, False), (1 << 20, 4, 10, True),
+                                                   (3000001, 0, 12, False), (2000000, 9, 7, True)])
+def test_partition_pass_one(pj, oracle, torch, n, shift, bits, with_pay):
+    rng = np.random.default_rng(n + bits)
+    keys = rng.integers(-2**31, 2**31 - 1, n).astype(np.int32)
+    pay = rng.integers(0, 2**31 - 1, n).astype(np.int32) if with_pay else None
+    dk = torch.from_numpy(keys).cuda()
+    dp = torch.from_numpy(pay).cuda() if with_pay else None
+    tup, off = pj.partition_input_tables_pass_one(dk, dp, bits=bits, shift=shift)
+    k, p = tuples_np(tup, 4)
+    oracle.check_partitioned(keys, pay, k, p, off.cpu().numpy().astype(np.int64), shift, bits)
+    assert int(off[-1]) == n
+
+
+def test_partition_pass_one_int64(pj, oracle, torch):
+    rng = np.random.default_rng(3)
+    keys = rng.integers(-2**63, 2**63 - 1, 400000, dtype=np.int64)
+    tup, off = pj.partition_input_tables_pass_one(torch.from_numpy(keys).cuda(), None, bits=9, shift=3)
+    k, p = tuples_np(tup, 8)
+    oracle.check_partitioned(keys, None, k, p, off.cpu().numpy().astype(np.int64), 3, 9)
+
+
+def test_partition_pass_one_skewed(pj, oracle, torch):
+    n = 1 << 21
+    keys = np.full(n, 77, dtype=np.int32)
+    keys[::3] = np.arange(len(keys[::3]), dtype=np.int32)
+    tup, off = pj.partition_input_tables_pass_one(torch.from_numpy(keys).cuda(), None, bits=8)
+    k, p = tuples_np(tup, 4)
+    oracle.check_partitioned(keys, None, k, p, off.cpu().numpy().astype(np.int64), 0, 8)
+
+
+@pytest.mark.parametrize("n,b1,b2,kb", [(1000, 3, 2, 4), (1 << 20, 8, 8, 4), (777777, 6, 9, 4), (300000, 5, 5, 8)])
+def test_partition_pass_two(pj, oracle, torch, n, b1, b2, kb):
+    rng = np.random.default_rng(n)
+    dt = np.int32 if kb == 4 else np.int64
+    keys = rng.integers(np.iinfo(dt).min, np.iinfo(dt).max, n, dtype=dt)
+    t1, off1 = pj.partition_input_tables_pass_one(torch.from_numpy(keys).cuda(), None, bits=b1, shift=b2)
+    t2, off2 = pj.partition_input_tables_pass_two(t1, kb, b1, b2, off1)
+    k, p = tuples_np(t2, kb)
+    oracle.check_partitioned(keys, None, k, p, off2.cpu().numpy().astype(np.int64), 0, b1 + b2)
+
+
+def test_assign_blocks(pj, oracle, torch):
+    from tests.test_oracle import emulate_reference_block_assignment
+    rng = np.random.default_rng(12)
+    for P, base in [(1, 5), (17, 3), (4096, 100), (65536, 3072)]:
+        sizes = rng.integers(0, 2 * base, P).astype(np.int64)
+        sizes[rng.integers(0, P, max(1, P // 100))] *= 20  # heavy partitions
+        offsets = oracle.exclusive_scan(sizes)
+        total = int(sizes.sum())
+        scanned, pidx, tot = pj.handle_skewed_data(torch.from_numpy(offsets.astype(np.int32)).cuda(), total, P, base)
+        e_scanned, e_pidx, e_nblk = emulate_reference_block_assignment(offsets.tolist(), total, base)
+        assert int(tot.item()) == e_nblk
+        assert u32(scanned).tolist() == e_scanned
+        assert u32(pidx)[:e_nblk].tolist() == e_pidx
+
+
+def test_shared_memory_hash_join_phase(pj, oracle, torch):
+    rng = np.random.default_rng(13)
+    nb, npr, B = 200000, 600000, 10
+    bk = rng.permutation(1 << 20)[:nb].astype(np.int32)
+    pk = rng.integers(0, 1 << 20, npr).astype(np.int32)
+    tb, ob = pj.partition_input_tables_pass_one(torch.from_numpy(bk).cuda(), None, bits=B)
+    tp, op = pj.partition_input_tables_pass_one(torch.from_numpy(pk).cuda(), None, bits=B)
+    base = oracle.base_build_size(nb, 1 << B)
+    pairs, n = pj.shared_memory_hash_join(tb, ob, tp, op, 4, B, base)
+    exp = oracle.inner_join_cpu(bk, pk)
+    assert n == len(exp)
+    assert np.array_equal(oracle.sort_pairs(pairs.cpu().numpy()), oracle.sort_pairs(exp))
