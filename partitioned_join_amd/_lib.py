@@ -91,6 +91,21 @@ SIGNATURES = {
 _LIB = None
 
 
+def hip_runtime_path() -> str:
+    """The HIP runtime libpj.so binds to: PyTorch's bundled libamdhip64 when
+    torch is a ROCm build (so the process has exactly one runtime), else the
+    system ROCm one."""
+    try:
+        import torch
+        p = os.path.join(os.path.dirname(torch.__file__), "lib", "libamdhip64.so")
+        if os.path.exists(p):
+            return p
+    except ImportError:
+        pass
+    rocm = os.environ.get("ROCM_PATH", "/opt/rocm")
+    return os.path.join(rocm, "lib", "libamdhip64.so")
+
+
 def load() -> ctypes.CDLL:
     """Load libpj.so; raise if it has not been built."""
     global _LIB
@@ -99,6 +114,9 @@ def load() -> ctypes.CDLL:
             raise RuntimeError(
                 f"libpj.so not found at {LIB_PATH}: build it with `make -C partitioned_join_amd` "
                 "(or __graft_entry__.build()); there is no CPU fallback")
+        # libpj.so leaves the HIP runtime symbols undefined; make the runtime the
+        # process uses globally visible before loading it.
+        ctypes.CDLL(hip_runtime_path(), mode=ctypes.RTLD_GLOBAL)
         L = ctypes.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(L, name)

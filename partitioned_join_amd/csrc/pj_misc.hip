@@ -197,10 +197,26 @@ __global__ __launch_bounds__(1024) void k_digit_hist(const K* keys, int64_t n, i
     if (s_bins[i]) atomicAdd(&hist[i], s_bins[i]);
 }
 
+// Wide digits (more bins than fit in LDS): one global atomic per key.
+template <typename K>
+__global__ void k_digit_hist_global(const K* keys, int64_t n, int stride, int shift, int bits, uint32_t* hist) {
+  const uint32_t mask = (1u << bits) - 1u;
+  PJ_GRID_STRIDE(i, n) atomicAdd(&hist[(uint32_t(uint64_t(keys[i * stride])) >> shift) & mask], 1u);
+}
+
 hipError_t digit_hist(const void* keys, int64_t n, int key_bytes, int stride, int shift, int bits, uint32_t* hist,
                       hipStream_t s) {
   hipError_t e = launch_fill_u32(hist, 0, int64_t(1) << bits, s);
   if (e != hipSuccess || n == 0) return e;
+  if (bits > 14) {
+    if (key_bytes == 4)
+      hipLaunchKernelGGL(k_digit_hist_global<uint32_t>, dim3(grid_for(n, 256)), dim3(256), 0, s,
+                         static_cast<const uint32_t*>(keys), n, stride, shift, bits, hist);
+    else
+      hipLaunchKernelGGL(k_digit_hist_global<uint64_t>, dim3(grid_for(n, 256)), dim3(256), 0, s,
+                         static_cast<const uint64_t*>(keys), n, stride, shift, bits, hist);
+    return hipGetLastError();
+  }
   const size_t lds = (size_t(1) << bits) * 4;
   const int g = grid_for(n, 1024 * 16, 1024);
   if (key_bytes == 4)

@@ -124,8 +124,9 @@ def test_edge_keys_and_sizes(pj, oracle, torch):
     extremes = np.array([-1, 0, 1, -2**31, 2**31 - 1, -2, 65535, 65536, -65536], dtype=np.int32)
     bk = np.concatenate([extremes, np.arange(1000, dtype=np.int32) * 65536])
     pk = np.concatenate([extremes, extremes, np.arange(2000, dtype=np.int32) * 32768])
+    # 0 and 65536 occur twice on the build side: non-unique oracle semantics
     for plan in [None, {"passes": 1, "bits1": 1}, {"passes": 2, "bits1": 8, "bits2": 8}]:
-        check_join(pj, oracle, torch, bk, pk, plan=plan)
+        check_join(pj, oracle, torch, bk, pk, plan=plan, uniq=False)
     one = np.array([7], dtype=np.int32)
     check_join(pj, oracle, torch, one, one)
     check_join(pj, oracle, torch, one, np.array([8], dtype=np.int32))
