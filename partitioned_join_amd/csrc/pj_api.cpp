@@ -82,6 +82,7 @@ static void carve_join(Carver& c, const PartParams& pp, Layout* L) {
   L->stray_lo = static_cast<uint64_t*>(c.take((size_t(kJoinGrid) + 3) * 8));
   L->stray_pre = static_cast<uint64_t*>(c.take((size_t(kJoinGrid) + 3) * 8));
   L->spill = static_cast<pj_pair*>(c.take(size_t(kJoinGrid) * size_t(kOutChunk) * 8));
+  L->scan_scratch = c.take(2 * 4096 * 8);
 }
 
 size_t layout_join(int64_t nR, int64_t nS, int key_bytes, const PartParams& pp, char* base, Layout* L) {
@@ -122,11 +123,15 @@ static pj_status check_pair(const pj_table* b, const pj_table* p, bool need_data
   return PJ_OK;
 }
 
+// Build chunks of f = 1.5 times the average partition: the reference defaults to
+// 3.0 (main.cu:96) to absorb skew inside one block, here heavy partitions are
+// split on both sides instead, and the smaller table lets two join workgroups
+// share a CU's LDS.
 static void plain_plan(int64_t nR, int key_bytes, pj_plan* plan) {
   std::memset(plan, 0, sizeof(*plan));
-  const double f = 3.0;
+  const double f = 1.5;
   const double cap = double(join_table_capacity(key_bytes) / 2 - 1);
-  const double target = cap < 3072.0 ? cap : 3072.0;  // build tuples per chunk at factor f
+  const double target = cap < 1536.0 ? cap : 1536.0;  // build tuples per chunk at factor f
   int B = 1;
   while (B < kMaxTotalBits && f * double(nR) / double(int64_t(1) << B) > target) ++B;
   plan->max_partition_size_factor = f;
@@ -351,13 +356,14 @@ static void one_table_layout(Carver& c, const pj_table* t, const PartParams& pp,
   T.nseg = choose_nseg(t->n);
   T.mat = static_cast<uint32_t*>(c.take(size_t(T.nseg) * pp.P * 4));
   T.matc = pp.passes == 2 ? static_cast<uint32_t*>(c.take(size_t(T.nseg) * pp.F1 * 4)) : nullptr;
+  L->scan_scratch = c.take(2 * 4096 * 8);
   L->t[1] = T;  // unused second slot mirrors the first
 }
 
 size_t pj_histogram_workspace_size(int32_t total_bits) {
   if (total_bits < 1 || total_bits > kMaxTotalBits) return 0;
   const size_t P = size_t(1) << total_bits;
-  return size_t(kMaxSegPerXcd * kNumXcd) * (P + 4096) * 4 + 4 * 256 + P * 4;
+  return size_t(kMaxSegPerXcd * kNumXcd) * (P + 4096) * 4 + 4 * 256 + P * 4 + 2 * 4096 * 8;
 }
 
 pj_status pj_combined_histogram(const pj_table* table, int32_t total_bits, int32_t bits2, uint32_t* d_fine,
@@ -434,7 +440,7 @@ pj_status pj_inclusive_scan(const uint32_t* d_in, uint32_t* d_out, int64_t n, vo
 size_t pj_partition_workspace_size(const pj_table* table, int32_t bits) {
   if (!table || bits < 1 || bits > 12 || table->n < 0) return 0;
   const size_t P = size_t(1) << bits;
-  return size_t(choose_nseg(table->n)) * P * 4 + P * 4 + 1024;
+  return size_t(choose_nseg(table->n)) * P * 4 + P * 4 + 2 * 4096 * 8 + 1024;
 }
 
 pj_status pj_partition_pass(const pj_table* table, int32_t shift, int32_t bits, void* d_tuples_out,
@@ -473,7 +479,7 @@ size_t pj_partition2_workspace_size(int64_t n, int32_t bits1, int32_t bits2) {
   (void)n;
   if (bits1 < 1 || bits1 > 12 || bits2 < 1 || bits2 > 12 || bits1 + bits2 > kMaxTotalBits) return 0;
   const size_t P = size_t(1) << (bits1 + bits2), F1 = size_t(1) << bits1;
-  return size_t(kNumXcd) * (P + F1) * 4 + P * 4 + 4 * 256;
+  return size_t(kNumXcd) * (P + F1) * 4 + P * 4 + 2 * 4096 * 8 + 4 * 256;
 }
 
 pj_status pj_partition_pass2(const void* d_tuples_in, int64_t n, int32_t key_bytes, int32_t bits1, int32_t bits2,
@@ -508,6 +514,7 @@ pj_status pj_partition_pass2(const void* d_tuples_in, int64_t n, int32_t key_byt
   T.matc = static_cast<uint32_t*>(c.take(size_t(kNumXcd) * pp.F1 * 4));
   T.tot = static_cast<uint32_t*>(c.take(size_t(pp.P) * 4));
   T.off = d_offsets2;
+  L.scan_scratch = c.take(2 * 4096 * 8);
   L.t[1] = T;
   PJ_HIP(digit_hist(d_tuples_in, n, key_bytes, 2, 0, pp.B, T.tot, stream));
   PJ_HIP(launch_offsets(L, stream));

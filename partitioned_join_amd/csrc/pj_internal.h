@@ -92,6 +92,7 @@ struct Layout {
   uint64_t* hole_pre;  // [kJoinGrid + 3]
   uint64_t* stray_lo;  // [kJoinGrid + 2]
   uint64_t* stray_pre; // [kJoinGrid + 3]
+  void* scan_scratch;  // block totals of the two-launch scans (2 x 4096 x 8 bytes)
   size_t bytes;
 };
 

@@ -27,35 +27,8 @@ namespace pj {
 
 int join_table_capacity(int key_bytes) { return key_bytes == 4 ? 7936 : 5120; }
 
-static size_t join_lds_bytes(int key_bytes, int table_slots) {
-  return size_t(table_slots) * (key_bytes == 4 ? 8 : 12) + size_t(kJoinStage) * 8 + 256;
-}
-
-// ---------------------------------------------------------------------------
-// k_join_plan: item_scan[q] = exclusive scan of items per partition;
-// item_scan[P] = total.  Also resets the output chunk counter.
-// ---------------------------------------------------------------------------
-__global__ __launch_bounds__(1024) void k_join_plan(const uint32_t* roff, const uint32_t* soff, int P, int base,
-                                                    int64_t slice, uint64_t* item_scan, uint32_t* chunk_ctr) {
-  __shared__ uint64_t s_tmp[1024 / kWave + 1];
-  const int per = (P + 1023) / 1024;
-  const int a = threadIdx.x * per, e = min(P, a + per);
-  uint64_t sum = 0;
-  for (int q = a; q < e; ++q) {
-    const uint64_t nr = roff[q + 1] - roff[q], ns = soff[q + 1] - soff[q];
-    if (nr && ns) sum += ((nr + base - 1) / base) * ((ns + slice - 1) / slice);
-  }
-  uint64_t total;
-  uint64_t run = block_exclusive_scan64<1024>(sum, s_tmp, &total);
-  for (int q = a; q < e; ++q) {
-    const uint64_t nr = roff[q + 1] - roff[q], ns = soff[q + 1] - soff[q];
-    item_scan[q] = run;
-    if (nr && ns) run += ((nr + base - 1) / base) * ((ns + slice - 1) / slice);
-  }
-  if (threadIdx.x == 0) {
-    item_scan[P] = total;
-    *chunk_ctr = 0;
-  }
+static size_t join_lds_bytes(int key_bytes, int table_slots, int stage_pairs) {
+  return size_t(table_slots) * (key_bytes == 4 ? 8 : 12) + size_t(stage_pairs) * 8 + 256;
 }
 
 // ---------------------------------------------------------------------------
@@ -79,7 +52,9 @@ __device__ __forceinline__ void store_pair(const JoinArgs& A, uint64_t v, uint2 
   }
 }
 
-// Called by all threads after a barrier (ctl->n stable).  Ends with a barrier.
+// Called by all threads after a barrier (ctl->n stable).  Writes the staged
+// pairs to the workgroup's output chunk, claiming a new chunk with one global
+// atomic when the current one fills.  Two barriers; ends with one.
 template <int BLOCK>
 __device__ __forceinline__ void flush_stage(const JoinArgs& A, StageCtl* ctl, const uint2* s_stage) {
   const uint64_t spill_cap = uint64_t(gridDim.x) * uint64_t(kOutChunk);
@@ -89,6 +64,7 @@ __device__ __forceinline__ void flush_stage(const JoinArgs& A, StageCtl* ctl, co
     const uint32_t a = n < avail ? n : avail;
     ctl->dst0 = uint64_t(ctl->chunk) * uint64_t(kOutChunk) + ctl->fill;
     ctl->a = a;
+    ctl->pad = n;
     if (n > a) {
       const uint32_t c2 = atomicAdd(A.chunk_ctr, 1u);
       ctl->dst1 = uint64_t(c2) * uint64_t(kOutChunk);
@@ -98,16 +74,15 @@ __device__ __forceinline__ void flush_stage(const JoinArgs& A, StageCtl* ctl, co
     } else {
       ctl->fill += a;
     }
+    ctl->n = 0;  // the next additions come after the closing barrier
   }
   __syncthreads();
-  const uint32_t n = ctl->n, a = ctl->a;
+  const uint32_t n = ctl->pad, a = ctl->a;
   const uint64_t d0 = ctl->dst0, d1 = ctl->dst1;
   for (uint32_t i = threadIdx.x; i < n; i += BLOCK) {
     const uint64_t v = i < a ? d0 + i : d1 + (i - a);
     store_pair(A, v, s_stage[i], spill_cap);
   }
-  __syncthreads();
-  if (threadIdx.x == 0) ctl->n = 0;
   __syncthreads();
 }
 
@@ -117,24 +92,82 @@ __device__ __forceinline__ uint64_t lds_cas(uint64_t* p, uint64_t cmp, uint64_t 
                             (unsigned long long)val));
 }
 
+// One work item: partition q, build chunk [rlo, rhi), probe slice [slo, shi).
+struct ItemDesc {
+  uint32_t q, rlo, rhi, slo, shi, valid;
+  FastDiv fd;  // table size 2 * (rhi - rlo) + 2
+};
+
+// Decodes work item `item` (wave 0 only): a 64-ary search over item_scan
+// (3 dependent loads for 65536 partitions), then the partition bounds.
+__device__ __forceinline__ void decode_item(const JoinArgs& A, uint64_t item, uint64_t total, ItemDesc* out) {
+  const uint32_t lane = lane_id();
+  if (item >= total) {
+    if (lane == 0) out->valid = 0;
+    return;
+  }
+  uint32_t lo = 0, hi = uint32_t(A.P);  // item_scan[lo] <= item < item_scan[hi]
+  while (hi - lo > 1) {
+    const uint32_t step = (hi - lo + 63) / 64;
+    const uint32_t idx = lo + lane * step;
+    const uint64_t v = idx < hi ? A.item_scan[idx] : ~0ull;
+    const uint64_t m = __ballot(v <= item);
+    const uint32_t k = 63 - __clzll((long long)m);
+    lo = lo + k * step;
+    hi = min(lo + step, hi);
+  }
+  const uint32_t q = lo;
+  uint64_t v = 0;
+  if (lane == 0) v = A.item_scan[q];
+  else if (lane == 1) v = A.roff[q];
+  else if (lane == 2) v = A.roff[q + 1];
+  else if (lane == 3) v = A.soff[q];
+  else if (lane == 4) v = A.soff[q + 1];
+  const uint64_t scan_q = __shfl(v, 0, kWave);
+  const uint32_t r0 = uint32_t(__shfl(v, 1, kWave)), r1 = min(uint32_t(__shfl(v, 2, kWave)), A.nr);
+  const uint32_t s0 = uint32_t(__shfl(v, 3, kWave)), s1 = min(uint32_t(__shfl(v, 4, kWave)), A.ns);
+  if (lane == 0) {
+    const uint64_t local = item - scan_q;
+    const uint64_t ns = (uint64_t(s1 - s0) + A.slice - 1) / A.slice;
+    const uint64_t c = local / ns, sl = local % ns;
+    const uint32_t rlo = min(r1, r0 + uint32_t(c) * uint32_t(A.base));
+    const uint32_t rhi = min(r1, rlo + uint32_t(A.base));
+    const uint64_t slo64 = uint64_t(s0) + sl * uint64_t(A.slice);
+    out->q = q;
+    out->rlo = rlo;
+    out->rhi = rhi;
+    out->slo = uint32_t(min(uint64_t(s1), slo64));
+    out->shi = uint32_t(min(uint64_t(s1), slo64 + uint64_t(A.slice)));
+    out->fd = fastdiv_make(int32_t(2 * (rhi - rlo) + 2));
+    out->valid = 1;
+  }
+}
+
 // ---------------------------------------------------------------------------
-// k_join
+// k_join: software-pipelined persistent join.  While item i is built and
+// probed, wave 0 decodes item i + grid and every thread has that item's build
+// tuples and first probe round already in flight.  Each thread holds up to NB
+// build and R probe tuples; a round of R probes per thread runs as iterations
+// in which every pending probe advances to its next match (one match per probe
+// per iteration) and matches are appended to the LDS stage with one LDS atomic
+// per wave (wave prefix sum).
 // ---------------------------------------------------------------------------
-template <int KB, int BLOCK, bool UNIQUE>
+template <int KB, int BLOCK, int NB, int R, bool UNIQUE>
 __global__ __launch_bounds__(BLOCK) void k_join(JoinArgs A) {
   using KTr = KeyT<KB>;
   using key_t = typename KTr::key_t;
   using tup_t = typename KTr::tup_t;
+  constexpr uint32_t ROUND = uint32_t(R) * BLOCK;
   extern __shared__ __align__(16) uint8_t s_raw[];
-  uint2* s_stage = reinterpret_cast<uint2*>(s_raw);
-  key_t* s_tkeys = reinterpret_cast<key_t*>(s_raw + size_t(kJoinStage) * 8);
+  uint2* s_stage = reinterpret_cast<uint2*>(s_raw);  // ROUND pairs
+  key_t* s_tkeys = reinterpret_cast<key_t*>(s_raw + size_t(ROUND) * 8);
   int32_t* s_tpays = reinterpret_cast<int32_t*>(s_tkeys + A.table_cap);
   __shared__ StageCtl ctl;
-  __shared__ uint32_t s_rlo, s_rhi, s_slo, s_shi, s_q;
-  __shared__ FastDiv s_fd;
+  __shared__ ItemDesc s_desc[2];
 
   const int tid = threadIdx.x;
   const uint32_t lane = lane_id();
+  const bool wave0 = tid < kWave;
   const tup_t* rt = static_cast<const tup_t*>(A.rt);
   const tup_t* st = static_cast<const tup_t*>(A.st);
   const uint64_t total_items = A.item_scan[A.P];
@@ -144,93 +177,131 @@ __global__ __launch_bounds__(BLOCK) void k_join(JoinArgs A) {
     ctl.chunk = 0;
     ctl.has_chunk = 0;
   }
+  uint64_t item = blockIdx.x;
+  if (wave0) decode_item(A, item, total_items, &s_desc[0]);
   __syncthreads();
 
-  for (uint64_t item = blockIdx.x; item < total_items; item += gridDim.x) {
-    if (tid == 0) {
-      int lo = 0, hi = A.P;  // item_scan[lo] <= item < item_scan[hi]
-      while (hi - lo > 1) {
-        const int mid = (lo + hi) >> 1;
-        if (A.item_scan[mid] <= item) lo = mid; else hi = mid;
-      }
-      const int q = lo;
-      const uint64_t local = item - A.item_scan[q];
-      const uint32_t r0 = A.roff[q], r1 = min(A.roff[q + 1], A.nr), s0 = A.soff[q], s1 = min(A.soff[q + 1], A.ns);
-      const uint64_t ns = (uint64_t(s1 - s0) + A.slice - 1) / A.slice;
-      const uint64_t c = local / ns, sl = local % ns;
-      const uint32_t rlo = min(r1, r0 + uint32_t(c) * uint32_t(A.base));
-      const uint32_t rhi = min(r1, rlo + uint32_t(A.base));
-      const uint64_t slo64 = uint64_t(s0) + sl * uint64_t(A.slice);
-      const uint32_t slo = uint32_t(min(uint64_t(s1), slo64));
-      const uint32_t shi = uint32_t(min(uint64_t(s1), slo64 + uint64_t(A.slice)));
-      s_rlo = rlo; s_rhi = rhi; s_slo = slo; s_shi = shi; s_q = uint32_t(q);
-      s_fd = fastdiv_make(int32_t(2 * (rhi - rlo) + 2));
+  ItemDesc d = s_desc[0];
+  int par = 0;
+  tup_t bt[NB], pt[R];
+  auto load_item = [&](const ItemDesc& x, tup_t (&b)[NB], tup_t (&p)[R]) {
+#pragma unroll
+    for (int k = 0; k < NB; ++k) {
+      const uint32_t i = x.rlo + uint32_t(k) * BLOCK + tid;
+      if (i < x.rhi) b[k] = rt[i];
     }
-    __syncthreads();
-    const uint32_t rlo = s_rlo, rhi = s_rhi, slo = s_slo, shi = s_shi;
-    const FastDiv fd = s_fd;
-    const uint32_t tsize = uint32_t(fd.d);
-    const key_t sentinel = key_t(s_q ^ 1u);
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+      const uint32_t j = x.slo + uint32_t(k) * BLOCK + tid;
+      if (j < x.shi) p[k] = st[j];
+    }
+  };
+  if (d.valid) load_item(d, bt, pt);
+
+  while (d.valid) {
+    const uint64_t nxt = item + gridDim.x;
+    if (wave0) decode_item(A, nxt, total_items, &s_desc[par ^ 1]);
+    const uint32_t tsize = uint32_t(d.fd.d);
+    const key_t sentinel = key_t(d.q ^ 1u);
     for (uint32_t i = tid; i < tsize; i += BLOCK) s_tkeys[i] = sentinel;
     __syncthreads();
+    const ItemDesc dn = s_desc[par ^ 1];
+    tup_t bn[NB], pn[R];
+    if (dn.valid) load_item(dn, bn, pn);
     // Build.
-    for (uint32_t i = rlo + tid; i < rhi; i += BLOCK) {
-      const tup_t t = rt[i];
-      const key_t k = KTr::key(t);
-      uint32_t slot = hash_slot(murmur3(k), fd);
-      while (true) {
-        const key_t old = lds_cas(&s_tkeys[slot], sentinel, k);
-        if (old == sentinel) {
-          s_tpays[slot] = KTr::pay(t);
-          break;
+#pragma unroll
+    for (int k = 0; k < NB; ++k) {
+      const uint32_t i = d.rlo + uint32_t(k) * BLOCK + tid;
+      if (i < d.rhi) {
+        const key_t key = KTr::key(bt[k]);
+        uint32_t slot = hash_slot(murmur3(key), d.fd);
+        while (true) {
+          const key_t old = lds_cas(&s_tkeys[slot], sentinel, key);
+          if (old == sentinel) {
+            s_tpays[slot] = KTr::pay(bt[k]);
+            break;
+          }
+          slot = (slot + 1 == tsize) ? 0u : slot + 1;
         }
-        slot = (slot + 1 == tsize) ? 0u : slot + 1;
       }
     }
     __syncthreads();
-    // Probe, one tuple per thread per round.
-    for (uint32_t r0 = slo; r0 < shi; r0 += BLOCK) {
-      const uint32_t j = r0 + tid;
-      bool pending = j < shi;
-      key_t k = 0;
-      int32_t ppay = 0;
-      uint32_t slot = 0;
-      if (pending) {
-        const tup_t t = st[j];
-        k = KTr::key(t);
-        ppay = KTr::pay(t);
-        slot = hash_slot(murmur3(k), fd);
+    // Probe rounds of R tuples per thread; the first round is already in registers.
+    for (uint32_t r0 = d.slo; r0 < d.shi; r0 += ROUND) {
+      if (r0 != d.slo) {
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+          const uint32_t j = r0 + uint32_t(k) * BLOCK + tid;
+          if (j < d.shi) pt[k] = st[j];
+        }
+      }
+      uint32_t slot[R];
+      bool pend[R];
+#pragma unroll
+      for (int k = 0; k < R; ++k) {
+        const uint32_t j = r0 + uint32_t(k) * BLOCK + tid;
+        pend[k] = j < d.shi;
+        slot[k] = pend[k] ? hash_slot(murmur3(KTr::key(pt[k])), d.fd) : 0u;
       }
       while (true) {
-        bool has = false;
-        int32_t bpay = 0;
-        if (pending) {
-          while (true) {
-            const key_t kk = s_tkeys[slot];
-            if (kk == sentinel) { pending = false; break; }
-            if (kk == k) { has = true; bpay = s_tpays[slot]; break; }
-            slot = (slot + 1 == tsize) ? 0u : slot + 1;
+        int32_t mb[R];
+        bool has[R];
+        uint32_t cnt = 0;
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+          has[k] = false;
+          mb[k] = 0;
+          if (pend[k]) {
+            const key_t key = KTr::key(pt[k]);
+            uint32_t s = slot[k];
+            while (true) {
+              const key_t kk = s_tkeys[s];
+              if (kk == sentinel) { pend[k] = false; break; }
+              if (kk == key) {
+                has[k] = true;
+                mb[k] = s_tpays[s];
+                if (UNIQUE) pend[k] = false;
+                s = (s + 1 == tsize) ? 0u : s + 1;
+                break;
+              }
+              s = (s + 1 == tsize) ? 0u : s + 1;
+            }
+            slot[k] = s;
+            cnt += has[k] ? 1u : 0u;
           }
         }
-        const uint64_t hm = __ballot(has);
-        uint32_t wbase = 0;
-        if (lane == 0 && hm) wbase = atomicAdd(&ctl.n, uint32_t(__popcll(hm)));
-        wbase = __shfl(wbase, 0, kWave);
-        if (has) {
-          s_stage[wbase + mbcnt(hm)] = make_uint2(uint32_t(bpay), uint32_t(ppay));
-          if (UNIQUE) pending = false;
-          else slot = (slot + 1 == tsize) ? 0u : slot + 1;
+        // wave prefix sum of the per-lane match counts; one LDS atomic per wave
+        uint32_t incl = cnt;
+#pragma unroll
+        for (int dlt = 1; dlt < kWave; dlt <<= 1) {
+          const uint32_t y = __shfl_up(incl, dlt, kWave);
+          if (lane >= uint32_t(dlt)) incl += y;
         }
-        const int any = __syncthreads_or(pending);
-        const uint32_t staged = ctl.n;
-        __syncthreads();  // every wave has read ctl.n before the next round adds to it
-        if (staged > uint32_t(kJoinStage - BLOCK)) flush_stage<BLOCK>(A, &ctl, s_stage);
+        const uint32_t wtotal = __shfl(incl, kWave - 1, kWave);
+        uint32_t wbase = 0;
+        if (lane == 0 && wtotal) wbase = atomicAdd(&ctl.n, wtotal);
+        uint32_t pos = __shfl(wbase, 0, kWave) + incl - cnt;
+#pragma unroll
+        for (int k = 0; k < R; ++k)
+          if (has[k]) s_stage[pos++] = make_uint2(uint32_t(mb[k]), uint32_t(KTr::pay(pt[k])));
+        bool anyp = false;
+#pragma unroll
+        for (int k = 0; k < R; ++k) anyp |= pend[k];
+        const int any = __syncthreads_or(anyp);
+        if (ctl.n > 0) flush_stage<BLOCK>(A, &ctl, s_stage);
         if (!any) break;
       }
     }
     __syncthreads();
+    // advance the pipeline
+    d = dn;
+#pragma unroll
+    for (int k = 0; k < NB; ++k) bt[k] = bn[k];
+#pragma unroll
+    for (int k = 0; k < R; ++k) pt[k] = pn[k];
+    par ^= 1;
+    item = nxt;
   }
-  if (ctl.n > 0) flush_stage<BLOCK>(A, &ctl, s_stage);
   if (tid == 0)
     A.wg_last[blockIdx.x] = ctl.has_chunk ? ((uint64_t(ctl.chunk) << 32) | ctl.fill) : ~0ull;
 }
@@ -247,78 +318,73 @@ __global__ __launch_bounds__(1024) void k_defrag_plan(const uint32_t* chunk_ctr,
                                                       uint64_t* stray_pre) {
   __shared__ uint64_t s_rec[1024];
   __shared__ uint64_t s_sorted[1024];
-  __shared__ uint32_t s_w;
-  __shared__ uint64_t s_fill_sum;
+  __shared__ uint64_t s_tmp[1024 / kWave + 1];
   const int tid = threadIdx.x;
   const uint64_t CH = uint64_t(kOutChunk);
-  if (tid == 0) { s_w = 0; s_fill_sum = 0; }
-  __syncthreads();
-  uint64_t rec = tid < nwg ? wg_last[tid] : ~0ull;
+  const uint64_t rec = tid < nwg ? wg_last[tid] : ~0ull;
+  const bool valid = rec != ~0ull;
   s_rec[tid] = rec;
-  if (rec != ~0ull) {
-    atomicAdd(&s_w, 1u);
-    atomicAdd(reinterpret_cast<unsigned long long*>(&s_fill_sum), (unsigned long long)(rec & 0xffffffffull));
-  }
   __syncthreads();
-  // rank by chunk id (ids are unique); records sort by (chunk << 32 | fill)
-  if (rec != ~0ull) {
+  if (valid) {  // rank by chunk id (chunk ids are unique)
     uint32_t rank = 0;
     for (int w = 0; w < nwg; ++w) rank += (s_rec[w] != ~0ull && s_rec[w] < rec) ? 1u : 0u;
     s_sorted[rank] = rec;
   }
-  __syncthreads();
-  if (tid != 0) return;
+  uint64_t fill_sum;
+  block_exclusive_scan64<1024>(valid ? (rec & 0xffffffffull) : 0ull, s_tmp, &fill_sum);
+  const uint32_t W = uint32_t(__syncthreads_count(valid));
   const uint64_t C = *chunk_ctr;
-  const uint32_t W = s_w;
-  const uint64_t total = (C - W) * CH + s_fill_sum;
-  *d_count = int64_t(total);
-  st->total = total;
+  const uint64_t total = (C - W) * CH + fill_sum;
+  if (tid == 0) {
+    *d_count = int64_t(total);
+    st->total = total;
+  }
   if (total > uint64_t(capacity)) {
-    st->moves = 0;
-    st->n_holes = 0;
-    st->n_strays = 0;
-    hole_pre[0] = 0;
-    stray_pre[0] = 0;
+    if (tid == 0) {
+      st->moves = 0;
+      st->n_holes = 0;
+      st->n_strays = 0;
+      hole_pre[0] = 0;
+      stray_pre[0] = 0;
+    }
     return;
   }
-  uint32_t nh = 0, ns = 0;
-  uint64_t hsum = 0, ssum = 0;
-  for (uint32_t i = 0; i < W; ++i) {
-    const uint64_t c = s_sorted[i] >> 32, f = s_sorted[i] & 0xffffffffull;
-    const uint64_t lo = c * CH + f, hi = min((c + 1) * CH, total);
-    if (lo < hi) {
-      hole_lo[nh] = lo;
-      hole_pre[nh] = hsum;
-      hsum += hi - lo;
-      ++nh;
-    }
-  }
-  hole_pre[nh] = hsum;
-  uint64_t cursor = total;
-  const uint64_t end = C * CH;
-  for (uint32_t i = 0; i < W; ++i) {
+  // record i (sorted): data [c*CH, c*CH + f), hole [c*CH + f, (c+1)*CH)
+  uint64_t hlen = 0, hlo = 0, slen = 0, slo = 0;
+  const uint32_t i = uint32_t(tid);
+  if (i < W) {
     const uint64_t c = s_sorted[i] >> 32, f = s_sorted[i] & 0xffffffffull;
     const uint64_t plo = c * CH + f, phi = (c + 1) * CH;
-    if (phi <= total) continue;
-    const uint64_t l = plo > total ? plo : total;
-    if (l > cursor) {
-      stray_lo[ns] = cursor;
-      stray_pre[ns] = ssum;
-      ssum += l - cursor;
-      ++ns;
-    }
-    if (phi > cursor) cursor = phi;
+    hlo = plo;
+    hlen = plo < total ? min(phi, total) - plo : 0;
+    const uint64_t prev_phi = i == 0 ? 0 : ((s_sorted[i - 1] >> 32) + 1) * CH;
+    slo = prev_phi > total ? prev_phi : total;  // data gap before this record's hole, above total
+    slen = plo > slo ? plo - slo : 0;
+  } else if (i == W) {  // tail above the last hole
+    const uint64_t last_phi = W == 0 ? 0 : ((s_sorted[W - 1] >> 32) + 1) * CH;
+    slo = last_phi > total ? last_phi : total;
+    slen = C * CH > slo ? C * CH - slo : 0;
   }
-  if (end > cursor) {
-    stray_lo[ns] = cursor;
-    stray_pre[ns] = ssum;
-    ssum += end - cursor;
-    ++ns;
+  __syncthreads();
+  uint64_t hsum, ssum;
+  const uint64_t hpre = block_exclusive_scan64<1024>(hlen, s_tmp, &hsum);
+  __syncthreads();
+  const uint64_t spre = block_exclusive_scan64<1024>(slen, s_tmp, &ssum);
+  if (i < W) {
+    hole_lo[i] = hlo;
+    hole_pre[i] = hpre;
   }
-  stray_pre[ns] = ssum;
-  st->n_holes = nh;
-  st->n_strays = ns;
-  st->moves = hsum < ssum ? hsum : ssum;
+  if (i <= W) {
+    stray_lo[i] = slo;
+    stray_pre[i] = spre;
+  }
+  if (tid == 0) {
+    hole_pre[W] = hsum;
+    stray_pre[W + 1] = ssum;
+    st->n_holes = W;
+    st->n_strays = W + 1;
+    st->moves = hsum < ssum ? hsum : ssum;
+  }
 }
 
 __device__ __forceinline__ uint32_t upper_index(const uint64_t* pre, uint32_t n, uint64_t k) {
@@ -351,24 +417,29 @@ __global__ __launch_bounds__(256) void k_defrag_move(const DefragState* st, cons
 // ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
-hipError_t launch_join_plan(const Layout& L, int base, int64_t slice, hipStream_t s) {
-  hipLaunchKernelGGL(k_join_plan, dim3(1), dim3(1024), 0, s, L.t[0].off, L.t[1].off, L.pp.P, base, slice,
-                     L.item_scan, L.chunk_ctr);
-  return hipGetLastError();
+template <int KB, int NB, int R>
+static void join_launch_t(const JoinArgs& a, int unique, size_t lds, hipStream_t s) {
+  dim3 grid(kJoinGrid), block(kJoinBlock);
+  if (unique) hipLaunchKernelGGL((k_join<KB, kJoinBlock, NB, R, true>), grid, block, lds, s, a);
+  else hipLaunchKernelGGL((k_join<KB, kJoinBlock, NB, R, false>), grid, block, lds, s, a);
 }
 
 static hipError_t launch_join_kernel(const JoinArgs& A, int key_bytes, int base, int unique, hipStream_t s) {
   const int slots = 2 * base + 2;  // table of a chunk of c tuples has 2c + 2 slots
   JoinArgs a = A;
   a.table_cap = slots;
-  const size_t lds = join_lds_bytes(key_bytes, slots);
-  dim3 grid(kJoinGrid), block(kJoinBlock);
+  // build tuples per thread: NB * kJoinBlock >= base
+  const bool small = base <= 4 * kJoinBlock;
   if (key_bytes == 4) {
-    if (unique) hipLaunchKernelGGL((k_join<4, kJoinBlock, true>), grid, block, lds, s, a);
-    else hipLaunchKernelGGL((k_join<4, kJoinBlock, false>), grid, block, lds, s, a);
+    constexpr int R = 8;
+    const size_t lds = join_lds_bytes(4, slots, R * kJoinBlock);
+    if (small) join_launch_t<4, 4, R>(a, unique, lds, s);
+    else join_launch_t<4, 8, R>(a, unique, lds, s);
   } else {
-    if (unique) hipLaunchKernelGGL((k_join<8, kJoinBlock, true>), grid, block, lds, s, a);
-    else hipLaunchKernelGGL((k_join<8, kJoinBlock, false>), grid, block, lds, s, a);
+    constexpr int R = 4;
+    const size_t lds = join_lds_bytes(8, slots, R * kJoinBlock);
+    if (small) join_launch_t<8, 4, R>(a, unique, lds, s);
+    else join_launch_t<8, 8, R>(a, unique, lds, s);
   }
   return hipGetLastError();
 }
@@ -414,7 +485,8 @@ size_t join_generic_ws_bytes(int B) {
   b += size_t(kJoinGrid) * 8;                         // wg_last
   b += 4 * (size_t(kJoinGrid) + 3) * 8;               // hole/stray arrays
   b += size_t(kJoinGrid) * size_t(kOutChunk) * 8;     // spill
-  return b + 1024;
+  b += 2 * 4096 * 8;                                  // scan block totals
+  return b + 2048;
 }
 
 hipError_t launch_join_generic(const void* rt, const uint32_t* roff, const void* st, const uint32_t* soff,
@@ -444,6 +516,7 @@ hipError_t launch_join_generic(const void* rt, const uint32_t* roff, const void*
   L.stray_lo = reinterpret_cast<uint64_t*>(take((size_t(kJoinGrid) + 3) * 8));
   L.stray_pre = reinterpret_cast<uint64_t*>(take((size_t(kJoinGrid) + 3) * 8));
   L.spill = reinterpret_cast<pj_pair*>(take(size_t(kJoinGrid) * size_t(kOutChunk) * 8));
+  L.scan_scratch = take(2 * 4096 * 8);
   hipError_t e = launch_join_plan(L, base, slice, s);
   if (e != hipSuccess) return e;
   e = launch_join(L, key_bytes, base, slice, unique, out, capacity, s);

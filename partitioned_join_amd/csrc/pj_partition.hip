@@ -67,6 +67,10 @@ hipError_t launch_fill_u64(uint64_t* p, uint64_t v, int64_t n, hipStream_t s) {
 // ---------------------------------------------------------------------------
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
+constexpr int kXscanBlock = 256;
+constexpr int kXscanPer = 16;
+constexpr int kXscanTile = kXscanBlock * kXscanPer;
+
 template <int KB, bool VEC>
 __device__ __forceinline__ void load4_low(const void* keys, uint32_t i, uint32_t hi, uint32_t low[4],
                                           bool valid[4]) {
@@ -270,29 +274,116 @@ hipError_t launch_colscan(const Layout& L, hipStream_t s) {
 }
 
 // ---------------------------------------------------------------------------
-// k_offsets: off[q] = exclusive scan of tot over q, off[P] = n.
+// Two-launch exclusive scan of per-index values (partition offsets, join work
+// list).  k_xscan_local: each block scans kXscanTile consecutive values
+// (kXscanPer contiguous values per thread) and stores its total;
+// k_xscan_apply adds the sum of the preceding block totals and lets the value
+// type record the grand total.  blockIdx.y selects the table.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(1024) void k_offsets(PartTable t0, PartTable t1, PartParams pp) {
-  __shared__ uint32_t s_tmp[1024 / kWave + 1];
-  const PartTable T = blockIdx.y ? t1 : t0;
-  const int P = pp.P;
-  const int per = (P + 1023) / 1024;
-  const int a = threadIdx.x * per, e = min(P, a + per);
-  uint32_t sum = 0;
-  for (int i = a; i < e; ++i) sum += T.tot[i];
-  uint32_t total;
-  uint32_t run = block_exclusive_scan<1024>(sum, s_tmp, &total);
-  for (int i = a; i < e; ++i) {
-    const uint32_t v = T.tot[i];
-    T.off[i] = run;
-    run += v;
+template <typename T, class V>
+__global__ __launch_bounds__(kXscanBlock) void k_xscan_local(V v, int64_t n, T* bsum) {
+  __shared__ T s_tmp[kXscanBlock / kWave + 1];
+  const int y = blockIdx.y;
+  const int64_t base = int64_t(blockIdx.x) * kXscanTile + int64_t(threadIdx.x) * kXscanPer;
+  T vals[kXscanPer];
+  T sum = 0;
+#pragma unroll
+  for (int k = 0; k < kXscanPer; ++k) {
+    vals[k] = base + k < n ? T(v.value(y, base + k)) : T(0);
+    sum += vals[k];
   }
-  if (threadIdx.x == 0) T.off[P] = T.n;
+  T total;
+  T run;
+  if constexpr (sizeof(T) == 8) run = block_exclusive_scan64<kXscanBlock>(sum, s_tmp, &total);
+  else run = block_exclusive_scan<kXscanBlock>(sum, s_tmp, &total);
+  T* out = v.out[y];
+#pragma unroll
+  for (int k = 0; k < kXscanPer; ++k) {
+    if (base + k < n) out[base + k] = run;
+    run += vals[k];
+  }
+  if (threadIdx.x == 0) bsum[size_t(y) * gridDim.x + blockIdx.x] = total;
 }
 
-hipError_t launch_offsets(const Layout& L, hipStream_t s) {
-  hipLaunchKernelGGL(k_offsets, dim3(1, L.ntables), dim3(1024), 0, s, L.t[0], L.t[1], L.pp);
+template <typename T, class V>
+__global__ __launch_bounds__(kXscanBlock) void k_xscan_apply(V v, int64_t n, const T* bsum) {
+  __shared__ T s_red[kXscanBlock / kWave];
+  const int y = blockIdx.y;
+  const int nb = gridDim.x;
+  T pre = 0;  // sum of the preceding block totals (nb is small)
+  for (int b = threadIdx.x; b < int(blockIdx.x); b += kXscanBlock) pre += bsum[size_t(y) * nb + b];
+#pragma unroll
+  for (int dlt = kWave / 2; dlt > 0; dlt >>= 1) pre += __shfl_xor(pre, dlt, kWave);
+  if ((threadIdx.x & (kWave - 1)) == 0) s_red[threadIdx.x / kWave] = pre;
+  __syncthreads();
+  T offset = 0;
+#pragma unroll
+  for (int w = 0; w < kXscanBlock / kWave; ++w) offset += s_red[w];
+  T* out = v.out[y];
+  const int64_t base = int64_t(blockIdx.x) * kXscanTile;
+  for (int k = threadIdx.x; k < kXscanTile; k += kXscanBlock)
+    if (base + k < n) out[base + k] += offset;
+  if (blockIdx.x == nb - 1 && threadIdx.x == 0) v.finish(y, n, offset + bsum[size_t(y) * nb + blockIdx.x]);
+}
+
+template <typename T, class V>
+hipError_t launch_xscan(const V& v, int64_t n, int ntab, T* bsum, hipStream_t s) {
+  const int nb = int((n + kXscanTile - 1) / kXscanTile);
+  hipLaunchKernelGGL((k_xscan_local<T, V>), dim3(nb, ntab), dim3(kXscanBlock), 0, s, v, n, bsum);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL((k_xscan_apply<T, V>), dim3(nb, ntab), dim3(kXscanBlock), 0, s, v, n, bsum);
   return hipGetLastError();
+}
+
+// off[q] = exclusive scan of tot over q; off[P] = n.
+struct OffsetValues {
+  const uint32_t* tot[2];
+  uint32_t* out[2];
+  uint32_t rows[2];
+  __device__ uint32_t value(int y, int64_t i) const { return tot[y][i]; }
+  __device__ void finish(int y, int64_t n, uint32_t) const { out[y][n] = rows[y]; }
+};
+
+hipError_t launch_offsets(const Layout& L, hipStream_t s) {
+  OffsetValues v{};
+  for (int t = 0; t < L.ntables; ++t) {
+    v.tot[t] = L.t[t].tot;
+    v.out[t] = L.t[t].off;
+    v.rows[t] = L.t[t].n;
+  }
+  return launch_xscan<uint32_t>(v, L.pp.P, L.ntables, reinterpret_cast<uint32_t*>(L.scan_scratch), s);
+}
+
+// item_scan[q] = exclusive scan over partitions of ceil(|R_q|/base) * ceil(|S_q|/slice)
+// (0 when either side is empty); item_scan[P] = total; resets the output chunk counter.
+struct ItemValues {
+  const uint32_t* roff;
+  const uint32_t* soff;
+  uint64_t* out[2];
+  uint32_t* chunk_ctr;
+  uint32_t base;
+  uint64_t slice;
+  __device__ uint64_t value(int, int64_t q) const {
+    const uint64_t nr = roff[q + 1] - roff[q], ns = soff[q + 1] - soff[q];
+    return (nr && ns) ? ((nr + base - 1) / base) * ((ns + slice - 1) / slice) : 0ull;
+  }
+  __device__ void finish(int, int64_t n, uint64_t total) const {
+    out[0][n] = total;
+    *chunk_ctr = 0;
+  }
+};
+
+hipError_t launch_join_plan(const Layout& L, int base, int64_t slice, hipStream_t s) {
+  ItemValues v{};
+  v.roff = L.t[0].off;
+  v.soff = L.t[1].off;
+  v.out[0] = L.item_scan;
+  v.out[1] = L.item_scan;
+  v.chunk_ctr = L.chunk_ctr;
+  v.base = uint32_t(base);
+  v.slice = uint64_t(slice);
+  return launch_xscan<uint64_t>(v, L.pp.P, 1, reinterpret_cast<uint64_t*>(L.scan_scratch), s);
 }
 
 // ---------------------------------------------------------------------------

@@ -46,7 +46,7 @@ def test_plain_plans(pj):
     assert huge.bits1 + huge.bits2 == 16
     for n in [1, 100, 10**4, 10**6, 10**7, 10**8]:
         p = pj.plan_default(n, n, 8)
-        assert 1 <= p.bits1 <= 12 and p.max_partition_size_factor == 3.0 and p.source == 0
+        assert 1 <= p.bits1 <= 12 and p.max_partition_size_factor == 1.5 and p.source == 0
         assert p.passes == (1 if p.bits2 == 0 else 2)
 
 
