@@ -72,7 +72,7 @@ static void carve_table(Carver& c, int64_t n, int key_bytes, const PartParams& p
   T->off = static_cast<uint32_t*>(c.take(size_t(pp.P + 1) * 4));
 }
 
-static void carve_join(Carver& c, const PartParams& pp, Layout* L) {
+static void carve_join(Carver& c, const PartParams& pp, uint64_t desc_cap, Layout* L) {
   L->item_scan = static_cast<uint64_t*>(c.take(size_t(pp.P + 1) * 8));
   L->chunk_ctr = static_cast<uint32_t*>(c.take(64));
   L->dstate = static_cast<DefragState*>(c.take(sizeof(DefragState)));
@@ -83,9 +83,19 @@ static void carve_join(Carver& c, const PartParams& pp, Layout* L) {
   L->stray_pre = static_cast<uint64_t*>(c.take((size_t(kJoinGrid) + 3) * 8));
   L->spill = static_cast<pj_pair*>(c.take(size_t(kJoinGrid) * size_t(kOutChunk) * 8));
   L->scan_scratch = c.take(2 * 4096 * 8);
+  L->desc_cap = desc_cap;
+  L->desc = c.take(size_t(desc_cap) * 32);
 }
 
-size_t layout_join(int64_t nR, int64_t nS, int key_bytes, const PartParams& pp, char* base, Layout* L) {
+// Work item descriptors: enough for every partition plus every extra build
+// chunk and probe slice of a skewed table; items past the bound (only when
+// heavy keys sit on both sides) are decoded by search instead.
+static uint64_t desc_capacity(int64_t nR, int64_t nS, int P, int base, int64_t slice) {
+  return uint64_t(P) + uint64_t(nR / base) + uint64_t(nS / slice) + 4096;
+}
+
+size_t layout_join(int64_t nR, int64_t nS, int key_bytes, const PartParams& pp, int jbase, int64_t slice, char* base,
+                   Layout* L) {
   Carver c{base};
   Layout tmp{};
   Layout* Lp = L ? L : &tmp;
@@ -93,7 +103,7 @@ size_t layout_join(int64_t nR, int64_t nS, int key_bytes, const PartParams& pp, 
   Lp->pp = pp;
   carve_table(c, nR, key_bytes, pp, pp.passes == 2, &Lp->t[0]);
   carve_table(c, nS, key_bytes, pp, pp.passes == 2, &Lp->t[1]);
-  carve_join(c, pp, Lp);
+  carve_join(c, pp, desc_capacity(nR, nS, pp.P, jbase, slice), Lp);
   c.take(64);  // device pair count of the synchronous entry point
   Lp->bytes = c.off;
   return c.off;
@@ -232,19 +242,19 @@ static pj_status run_join(const pj_table* build, const pj_table* probe, const pj
     return PJ_OK;
   }
   const PartParams pp = params_of(&plan);
-  const size_t need = layout_join(build->n, probe->n, kb, pp, nullptr, nullptr);
+  const int base = join_base(build->n, pp.P, plan.max_partition_size_factor, kb);
+  const int64_t slice = join_slice(&plan, base);
+  const size_t need = layout_join(build->n, probe->n, kb, pp, base, slice, nullptr, nullptr);
   if (!workspace || ws_bytes < need)
     return fail(PJ_ERR_WORKSPACE, "workspace of %zu bytes is smaller than the %zu required", ws_bytes, need);
   if (reinterpret_cast<uintptr_t>(workspace) % 256 != 0)
     return fail(PJ_ERR_INVALID_ARG, "workspace must be 256-byte aligned");
   Layout L{};
-  layout_join(build->n, probe->n, kb, pp, static_cast<char*>(workspace), &L);
+  layout_join(build->n, probe->n, kb, pp, base, slice, static_cast<char*>(workspace), &L);
   L.t[0].keys = build->keys;
   L.t[0].pay = build->payload;
   L.t[1].keys = probe->keys;
   L.t[1].pay = probe->payload;
-  const int base = join_base(build->n, pp.P, plan.max_partition_size_factor, kb);
-  const int64_t slice = join_slice(&plan, base);
 
   if (g_prof.on && !g_prof.created) {
     for (auto& e : g_prof.ev) PJ_HIP(hipEventCreate(&e));
@@ -297,7 +307,9 @@ pj_status pj_workspace_size(const pj_table* build, const pj_table* probe, const 
   else plain_plan(build->n, build->key_bytes, &plan);
   st = check_plan(&plan);
   if (st != PJ_OK) return st;
-  *bytes = layout_join(build->n, probe->n, build->key_bytes, params_of(&plan), nullptr, nullptr);
+  const PartParams pp = params_of(&plan);
+  const int base = join_base(build->n, pp.P, plan.max_partition_size_factor, build->key_bytes);
+  *bytes = layout_join(build->n, probe->n, build->key_bytes, pp, base, join_slice(&plan, base), nullptr, nullptr);
   return PJ_OK;
 }
 

@@ -138,6 +138,26 @@ __device__ __forceinline__ uint32_t hash_slot(uint32_t h, const FastDiv& f) {
 }
 
 // ---------------------------------------------------------------------------
+// Barriers.  __syncthreads() carries a workgroup-scope fence, which waits for
+// every outstanding global load and store of the wave (vmcnt(0)) before the
+// barrier.  Where a barrier only orders LDS data, lds_barrier() waits for this
+// wave's LDS operations only, so prefetched global loads and write-backs stay
+// in flight across it.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void lds_barrier() { __asm__ volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
+// Block-wide OR of `pred` with one LDS barrier and no global fence.  `slots`
+// is 3 LDS words zeroed before first use; `round` counts calls (same on every
+// thread).
+__device__ __forceinline__ bool block_any(bool pred, uint32_t* slots, uint32_t round) {
+  const uint32_t cur = round % 3, nxt = (round + 1) % 3;
+  if (threadIdx.x == 0) slots[nxt] = 0;  // last read before the previous barrier
+  if (__ballot(pred) && (threadIdx.x & (kWave - 1)) == 0) slots[cur] = 1;
+  lds_barrier();
+  return slots[cur] != 0;
+}
+
+// ---------------------------------------------------------------------------
 // wave64 helpers
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
@@ -161,7 +181,7 @@ __device__ __forceinline__ uint64_t match_digit(uint32_t d, int bits, bool valid
 
 // Block-wide exclusive scan of one uint32 per thread.  s_tmp needs BLOCK/64 + 1
 // entries.  Returns the exclusive prefix; *total receives the block sum.
-// Contains two __syncthreads().
+// Contains two LDS barriers.
 template <int BLOCK>
 __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* s_tmp, uint32_t* total) {
   constexpr int NW = BLOCK / kWave;
@@ -173,7 +193,7 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* s
     if (lane >= uint32_t(d)) x += y;
   }
   if (lane == kWave - 1) s_tmp[wid] = x;
-  __syncthreads();
+  lds_barrier();
   if (wid == 0) {
     const uint32_t w = lane < uint32_t(NW) ? s_tmp[lane] : 0u;
     uint32_t wx = w;
@@ -185,7 +205,7 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* s
     if (lane < uint32_t(NW)) s_tmp[lane] = wx - w;
     if (lane == uint32_t(NW - 1)) s_tmp[NW] = wx;
   }
-  __syncthreads();
+  lds_barrier();
   const uint32_t res = x - v + s_tmp[wid];
   *total = s_tmp[NW];
   return res;
@@ -203,7 +223,7 @@ __device__ __forceinline__ uint64_t block_exclusive_scan64(uint64_t v, uint64_t*
     if (lane >= uint32_t(d)) x += y;
   }
   if (lane == kWave - 1) s_tmp[wid] = x;
-  __syncthreads();
+  lds_barrier();
   if (wid == 0) {
     const uint64_t w = lane < uint32_t(NW) ? s_tmp[lane] : 0ull;
     uint64_t wx = w;
@@ -215,7 +235,7 @@ __device__ __forceinline__ uint64_t block_exclusive_scan64(uint64_t v, uint64_t*
     if (lane < uint32_t(NW)) s_tmp[lane] = wx - w;
     if (lane == uint32_t(NW - 1)) s_tmp[NW] = wx;
   }
-  __syncthreads();
+  lds_barrier();
   const uint64_t res = x - v + s_tmp[wid];
   *total = s_tmp[NW];
   return res;

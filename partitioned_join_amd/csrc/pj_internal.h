@@ -69,6 +69,8 @@ struct JoinArgs {
   uint64_t* wg_last;       // [grid] (chunk << 32 | fill) or ~0 when no chunk; fill <= kOutChunk
   int table_cap;           // LDS slots available for the build table
   uint32_t nr, ns;         // table sizes (bounds for the item ranges)
+  const uint4* desc;       // expanded work item descriptors (2 uint4 per item)
+  uint64_t desc_cap;       // items with a descriptor; later items are decoded by search
 };
 
 struct DefragState {
@@ -93,12 +95,15 @@ struct Layout {
   uint64_t* stray_lo;  // [kJoinGrid + 2]
   uint64_t* stray_pre; // [kJoinGrid + 3]
   void* scan_scratch;  // block totals of the two-launch scans (2 x 4096 x 8 bytes)
+  void* desc;          // work item descriptors, 32 bytes each
+  uint64_t desc_cap;
   size_t bytes;
 };
 
 int tuple_bytes(int key_bytes);
 uint32_t choose_nseg(int64_t n);
-size_t layout_join(int64_t nR, int64_t nS, int key_bytes, const PartParams& pp, char* base, Layout* L);
+size_t layout_join(int64_t nR, int64_t nS, int key_bytes, const PartParams& pp, int base, int64_t slice,
+                   char* ws, Layout* L);
 
 // Launchers (return hipError_t of the launch).
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t s);

@@ -44,20 +44,26 @@ struct StageCtl {
   uint64_t dst0, dst1;
 };
 
-__device__ __forceinline__ void store_pair(const JoinArgs& A, uint64_t v, uint2 pr, uint64_t spill_cap) {
-  if (v < uint64_t(A.capacity)) {
-    reinterpret_cast<uint2*>(A.out)[v] = pr;
-  } else if (v - uint64_t(A.capacity) < spill_cap) {
-    reinterpret_cast<uint2*>(A.spill)[v - uint64_t(A.capacity)] = pr;
-  }
+// Output of the join: capacity pairs at out, then the spill area in the
+// workspace for chunks claimed past the capacity (compacted afterwards).
+struct OutSink {
+  uint2* out;
+  uint2* spill;
+  uint64_t capacity;
+  uint64_t spill_cap;
+  uint32_t* chunk_ctr;
+};
+
+__device__ __forceinline__ void store_pair(const OutSink o, uint64_t v, uint2 pr) {
+  if (v < o.capacity) o.out[v] = pr;
+  else if (v - o.capacity < o.spill_cap) o.spill[v - o.capacity] = pr;
 }
 
 // Called by all threads after a barrier (ctl->n stable).  Writes the staged
 // pairs to the workgroup's output chunk, claiming a new chunk with one global
 // atomic when the current one fills.  Two barriers; ends with one.
 template <int BLOCK>
-__device__ __forceinline__ void flush_stage(const JoinArgs& A, StageCtl* ctl, const uint2* s_stage) {
-  const uint64_t spill_cap = uint64_t(gridDim.x) * uint64_t(kOutChunk);
+__device__ __forceinline__ void flush_stage(const OutSink o, StageCtl* ctl, const uint2* s_stage) {
   if (threadIdx.x == 0) {
     const uint32_t n = ctl->n;
     const uint32_t avail = ctl->has_chunk ? uint32_t(kOutChunk) - ctl->fill : 0u;
@@ -66,7 +72,7 @@ __device__ __forceinline__ void flush_stage(const JoinArgs& A, StageCtl* ctl, co
     ctl->a = a;
     ctl->pad = n;
     if (n > a) {
-      const uint32_t c2 = atomicAdd(A.chunk_ctr, 1u);
+      const uint32_t c2 = atomicAdd(o.chunk_ctr, 1u);
       ctl->dst1 = uint64_t(c2) * uint64_t(kOutChunk);
       ctl->chunk = c2;
       ctl->fill = n - a;
@@ -76,14 +82,14 @@ __device__ __forceinline__ void flush_stage(const JoinArgs& A, StageCtl* ctl, co
     }
     ctl->n = 0;  // the next additions come after the closing barrier
   }
-  __syncthreads();
+  lds_barrier();
   const uint32_t n = ctl->pad, a = ctl->a;
   const uint64_t d0 = ctl->dst0, d1 = ctl->dst1;
   for (uint32_t i = threadIdx.x; i < n; i += BLOCK) {
     const uint64_t v = i < a ? d0 + i : d1 + (i - a);
-    store_pair(A, v, s_stage[i], spill_cap);
+    store_pair(o, v, s_stage[i]);
   }
-  __syncthreads();
+  lds_barrier();
 }
 
 __device__ __forceinline__ uint32_t lds_cas(uint32_t* p, uint32_t cmp, uint32_t val) { return atomicCAS(p, cmp, val); }
@@ -92,65 +98,92 @@ __device__ __forceinline__ uint64_t lds_cas(uint64_t* p, uint64_t cmp, uint64_t 
                             (unsigned long long)val));
 }
 
-// One work item: partition q, build chunk [rlo, rhi), probe slice [slo, shi).
+// One work item: partition q, build chunk [rlo, rhi), probe slice [slo, shi);
+// the LDS table of the chunk has fd.d = 2 * (rhi - rlo) + 2 slots.  Stored as
+// two uint4 words: {q, rlo, rhi, slo}, {shi, magic, shift, add_sign}.
 struct ItemDesc {
   uint32_t q, rlo, rhi, slo, shi, valid;
-  FastDiv fd;  // table size 2 * (rhi - rlo) + 2
+  FastDiv fd;
 };
 
-// Decodes work item `item` (wave 0 only): a 64-ary search over item_scan
-// (3 dependent loads for 65536 partitions), then the partition bounds.
-__device__ __forceinline__ void decode_item(const JoinArgs& A, uint64_t item, uint64_t total, ItemDesc* out) {
-  const uint32_t lane = lane_id();
-  if (item >= total) {
-    if (lane == 0) out->valid = 0;
-    return;
-  }
+// Per-thread decode of work item `item` < total: binary search of its
+// partition in item_scan, then its chunk and slice.
+__device__ __forceinline__ ItemDesc decode_item(const JoinArgs& A, uint64_t item) {
   uint32_t lo = 0, hi = uint32_t(A.P);  // item_scan[lo] <= item < item_scan[hi]
   while (hi - lo > 1) {
-    const uint32_t step = (hi - lo + 63) / 64;
-    const uint32_t idx = lo + lane * step;
-    const uint64_t v = idx < hi ? A.item_scan[idx] : ~0ull;
-    const uint64_t m = __ballot(v <= item);
-    const uint32_t k = 63 - __clzll((long long)m);
-    lo = lo + k * step;
-    hi = min(lo + step, hi);
+    const uint32_t mid = (lo + hi) >> 1;
+    if (A.item_scan[mid] <= item) lo = mid; else hi = mid;
   }
   const uint32_t q = lo;
-  uint64_t v = 0;
-  if (lane == 0) v = A.item_scan[q];
-  else if (lane == 1) v = A.roff[q];
-  else if (lane == 2) v = A.roff[q + 1];
-  else if (lane == 3) v = A.soff[q];
-  else if (lane == 4) v = A.soff[q + 1];
-  const uint64_t scan_q = __shfl(v, 0, kWave);
-  const uint32_t r0 = uint32_t(__shfl(v, 1, kWave)), r1 = min(uint32_t(__shfl(v, 2, kWave)), A.nr);
-  const uint32_t s0 = uint32_t(__shfl(v, 3, kWave)), s1 = min(uint32_t(__shfl(v, 4, kWave)), A.ns);
-  if (lane == 0) {
-    const uint64_t local = item - scan_q;
-    const uint64_t ns = (uint64_t(s1 - s0) + A.slice - 1) / A.slice;
-    const uint64_t c = local / ns, sl = local % ns;
-    const uint32_t rlo = min(r1, r0 + uint32_t(c) * uint32_t(A.base));
-    const uint32_t rhi = min(r1, rlo + uint32_t(A.base));
-    const uint64_t slo64 = uint64_t(s0) + sl * uint64_t(A.slice);
-    out->q = q;
-    out->rlo = rlo;
-    out->rhi = rhi;
-    out->slo = uint32_t(min(uint64_t(s1), slo64));
-    out->shi = uint32_t(min(uint64_t(s1), slo64 + uint64_t(A.slice)));
-    out->fd = fastdiv_make(int32_t(2 * (rhi - rlo) + 2));
-    out->valid = 1;
+  const uint64_t local = item - A.item_scan[q];
+  const uint32_t r0 = A.roff[q], r1 = min(A.roff[q + 1], A.nr);
+  const uint32_t s0 = A.soff[q], s1 = min(A.soff[q + 1], A.ns);
+  const uint64_t ns = (uint64_t(s1 - s0) + A.slice - 1) / A.slice;
+  const uint64_t c = local / ns, sl = local % ns;
+  ItemDesc d;
+  d.q = q;
+  d.rlo = min(r1, r0 + uint32_t(c) * uint32_t(A.base));
+  d.rhi = min(r1, d.rlo + uint32_t(A.base));
+  const uint64_t slo64 = uint64_t(s0) + sl * uint64_t(A.slice);
+  d.slo = uint32_t(min(uint64_t(s1), slo64));
+  d.shi = uint32_t(min(uint64_t(s1), slo64 + uint64_t(A.slice)));
+  d.fd = fastdiv_make(int32_t(2 * (d.rhi - d.rlo) + 2));
+  d.valid = 1;
+  return d;
+}
+
+// k_items_expand: descriptors of the first min(total, cap) work items, so the
+// join workgroups read one descriptor per item instead of searching.
+__global__ __launch_bounds__(256) void k_items_expand(JoinArgs A, uint4* desc, uint64_t cap) {
+  const uint64_t total = A.item_scan[A.P];
+  const uint64_t lim = total < cap ? total : cap;
+  for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < lim; i += uint64_t(gridDim.x) * blockDim.x) {
+    const ItemDesc d = decode_item(A, i);
+    desc[2 * i] = make_uint4(d.q, d.rlo, d.rhi, d.slo);
+    desc[2 * i + 1] = make_uint4(d.shi, uint32_t(d.fd.magic), uint32_t(d.fd.shift), uint32_t(d.fd.add_sign));
   }
+}
+
+// Raw descriptor words of `item`, loaded without a branch (the index is
+// clamped) so that nothing consumes them before desc_from_raw(): the loads
+// stay in flight across the current item.
+__device__ __forceinline__ void raw_desc_load(const JoinArgs& A, uint64_t item, uint4& a, uint4& b) {
+  const uint64_t i = item < A.desc_cap ? item : 0;
+  a = A.desc[2 * i];
+  b = A.desc[2 * i + 1];
+}
+
+__device__ __forceinline__ ItemDesc desc_from_raw(const JoinArgs& A, const uint4& a, const uint4& b, uint64_t item,
+                                                  uint64_t total) {
+  ItemDesc d;
+  if (item >= total) {
+    d.valid = 0;
+    d.q = d.rlo = d.rhi = d.slo = d.shi = 0;
+    d.fd = FastDiv{2, 0, -1, 0};
+    return d;
+  }
+  if (item >= A.desc_cap) return decode_item(A, item);  // beyond the expanded descriptors
+  d.q = a.x;
+  d.rlo = a.y;
+  d.rhi = a.z;
+  d.slo = a.w;
+  d.shi = b.x;
+  d.fd.d = int32_t(2 * (d.rhi - d.rlo) + 2);
+  d.fd.magic = int32_t(b.y);
+  d.fd.shift = int32_t(b.z);
+  d.fd.add_sign = int32_t(b.w);
+  d.valid = 1;
+  return d;
 }
 
 // ---------------------------------------------------------------------------
 // k_join: software-pipelined persistent join.  While item i is built and
-// probed, wave 0 decodes item i + grid and every thread has that item's build
-// tuples and first probe round already in flight.  Each thread holds up to NB
-// build and R probe tuples; a round of R probes per thread runs as iterations
-// in which every pending probe advances to its next match (one match per probe
-// per iteration) and matches are appended to the LDS stage with one LDS atomic
-// per wave (wave prefix sum).
+// probed, the build tuples and first probe round of item i + grid are in
+// flight into registers and the descriptor of item i + 2*grid is loading.
+// Each thread holds up to NB build and R probe tuples; a round of R probes per
+// thread runs as iterations in which every pending probe advances to its next
+// match (one match per probe per iteration) and matches are appended to the
+// LDS stage with one LDS atomic per wave (wave prefix sum).
 // ---------------------------------------------------------------------------
 template <int KB, int BLOCK, int NB, int R, bool UNIQUE>
 __global__ __launch_bounds__(BLOCK) void k_join(JoinArgs A) {
@@ -163,26 +196,31 @@ __global__ __launch_bounds__(BLOCK) void k_join(JoinArgs A) {
   key_t* s_tkeys = reinterpret_cast<key_t*>(s_raw + size_t(ROUND) * 8);
   int32_t* s_tpays = reinterpret_cast<int32_t*>(s_tkeys + A.table_cap);
   __shared__ StageCtl ctl;
-  __shared__ ItemDesc s_desc[2];
+  __shared__ uint32_t s_any[3];
+  uint32_t vote = 0;
 
   const int tid = threadIdx.x;
   const uint32_t lane = lane_id();
-  const bool wave0 = tid < kWave;
   const tup_t* rt = static_cast<const tup_t*>(A.rt);
   const tup_t* st = static_cast<const tup_t*>(A.st);
   const uint64_t total_items = A.item_scan[A.P];
+  const OutSink sink{reinterpret_cast<uint2*>(A.out), reinterpret_cast<uint2*>(A.spill), uint64_t(A.capacity),
+                     uint64_t(gridDim.x) * uint64_t(kOutChunk), A.chunk_ctr};
   if (tid == 0) {
     ctl.n = 0;
     ctl.fill = 0;
     ctl.chunk = 0;
     ctl.has_chunk = 0;
+    s_any[0] = s_any[1] = s_any[2] = 0;
   }
-  uint64_t item = blockIdx.x;
-  if (wave0) decode_item(A, item, total_items, &s_desc[0]);
-  __syncthreads();
+  lds_barrier();
 
-  ItemDesc d = s_desc[0];
-  int par = 0;
+  uint64_t item = blockIdx.x;
+  uint4 ra, rb;
+  raw_desc_load(A, item, ra, rb);
+  ItemDesc d = desc_from_raw(A, ra, rb, item, total_items);
+  raw_desc_load(A, item + gridDim.x, ra, rb);
+  ItemDesc dn = desc_from_raw(A, ra, rb, item + gridDim.x, total_items);
   tup_t bt[NB], pt[R];
   auto load_item = [&](const ItemDesc& x, tup_t (&b)[NB], tup_t (&p)[R]) {
 #pragma unroll
@@ -199,15 +237,14 @@ __global__ __launch_bounds__(BLOCK) void k_join(JoinArgs A) {
   if (d.valid) load_item(d, bt, pt);
 
   while (d.valid) {
-    const uint64_t nxt = item + gridDim.x;
-    if (wave0) decode_item(A, nxt, total_items, &s_desc[par ^ 1]);
+    tup_t bn[NB], pn[R];
+    if (dn.valid) load_item(dn, bn, pn);
+    const uint64_t item_nn = item + 2 * uint64_t(gridDim.x);
+    raw_desc_load(A, item_nn, ra, rb);  // decoded at the end of this item
     const uint32_t tsize = uint32_t(d.fd.d);
     const key_t sentinel = key_t(d.q ^ 1u);
     for (uint32_t i = tid; i < tsize; i += BLOCK) s_tkeys[i] = sentinel;
-    __syncthreads();
-    const ItemDesc dn = s_desc[par ^ 1];
-    tup_t bn[NB], pn[R];
-    if (dn.valid) load_item(dn, bn, pn);
+    lds_barrier();
     // Build.
 #pragma unroll
     for (int k = 0; k < NB; ++k) {
@@ -225,51 +262,38 @@ __global__ __launch_bounds__(BLOCK) void k_join(JoinArgs A) {
         }
       }
     }
-    __syncthreads();
-    // Probe rounds of R tuples per thread; the first round is already in registers.
-    for (uint32_t r0 = d.slo; r0 < d.shi; r0 += ROUND) {
-      if (r0 != d.slo) {
-#pragma unroll
-        for (int k = 0; k < R; ++k) {
-          const uint32_t j = r0 + uint32_t(k) * BLOCK + tid;
-          if (j < d.shi) pt[k] = st[j];
-        }
-      }
+    lds_barrier();
+    // Probe rounds of R tuples per thread.  The first round is already in
+    // registers; later rounds (slices longer than one round) load their own
+    // registers, so the first round's are never reloaded inside this item and
+    // need no wait that would also drain the next item's prefetch.
+    auto probe_round = [&](const tup_t (&ptr)[R], uint32_t r0) {
       uint32_t slot[R];
-      bool pend[R];
+      uint32_t pend = 0;  // bit k: probe k still walking
 #pragma unroll
       for (int k = 0; k < R; ++k) {
         const uint32_t j = r0 + uint32_t(k) * BLOCK + tid;
-        pend[k] = j < d.shi;
-        slot[k] = pend[k] ? hash_slot(murmur3(KTr::key(pt[k])), d.fd) : 0u;
+        const bool v = j < d.shi;
+        pend |= v ? (1u << k) : 0u;
+        slot[k] = v ? hash_slot(murmur3(KTr::key(ptr[k])), d.fd) : 0u;
       }
       while (true) {
-        int32_t mb[R];
-        bool has[R];
-        uint32_t cnt = 0;
+        uint32_t has = 0;  // bit k: probe k matched at slot[k] in this iteration
 #pragma unroll
         for (int k = 0; k < R; ++k) {
-          has[k] = false;
-          mb[k] = 0;
-          if (pend[k]) {
-            const key_t key = KTr::key(pt[k]);
+          if (pend & (1u << k)) {
+            const key_t key = KTr::key(ptr[k]);
             uint32_t s = slot[k];
             while (true) {
               const key_t kk = s_tkeys[s];
-              if (kk == sentinel) { pend[k] = false; break; }
-              if (kk == key) {
-                has[k] = true;
-                mb[k] = s_tpays[s];
-                if (UNIQUE) pend[k] = false;
-                s = (s + 1 == tsize) ? 0u : s + 1;
-                break;
-              }
+              if (kk == sentinel) { pend &= ~(1u << k); break; }
+              if (kk == key) { has |= 1u << k; break; }
               s = (s + 1 == tsize) ? 0u : s + 1;
             }
             slot[k] = s;
-            cnt += has[k] ? 1u : 0u;
           }
         }
+        const uint32_t cnt = __popc(has);
         // wave prefix sum of the per-lane match counts; one LDS atomic per wave
         uint32_t incl = cnt;
 #pragma unroll
@@ -283,24 +307,35 @@ __global__ __launch_bounds__(BLOCK) void k_join(JoinArgs A) {
         uint32_t pos = __shfl(wbase, 0, kWave) + incl - cnt;
 #pragma unroll
         for (int k = 0; k < R; ++k)
-          if (has[k]) s_stage[pos++] = make_uint2(uint32_t(mb[k]), uint32_t(KTr::pay(pt[k])));
-        bool anyp = false;
-#pragma unroll
-        for (int k = 0; k < R; ++k) anyp |= pend[k];
-        const int any = __syncthreads_or(anyp);
-        if (ctl.n > 0) flush_stage<BLOCK>(A, &ctl, s_stage);
+          if (has & (1u << k)) {
+            s_stage[pos++] = make_uint2(uint32_t(s_tpays[slot[k]]), uint32_t(KTr::pay(ptr[k])));
+            if (UNIQUE) pend &= ~(1u << k);
+            slot[k] = (slot[k] + 1 == tsize) ? 0u : slot[k] + 1;
+          }
+        const bool any = block_any(pend != 0, s_any, vote++);
+        if (ctl.n > 0) flush_stage<BLOCK>(sink, &ctl, s_stage);
         if (!any) break;
       }
+    };
+    if (d.slo < d.shi) probe_round(pt, d.slo);
+    for (uint32_t r0 = d.slo + ROUND; r0 < d.shi; r0 += ROUND) {
+      tup_t pr[R];
+#pragma unroll
+      for (int k = 0; k < R; ++k) {
+        const uint32_t j = r0 + uint32_t(k) * BLOCK + tid;
+        if (j < d.shi) pr[k] = st[j];
+      }
+      probe_round(pr, r0);
     }
-    __syncthreads();
+    lds_barrier();
     // advance the pipeline
     d = dn;
+    dn = desc_from_raw(A, ra, rb, item_nn, total_items);
 #pragma unroll
     for (int k = 0; k < NB; ++k) bt[k] = bn[k];
 #pragma unroll
     for (int k = 0; k < R; ++k) pt[k] = pn[k];
-    par ^= 1;
-    item = nxt;
+    item += gridDim.x;
   }
   if (tid == 0)
     A.wg_last[blockIdx.x] = ctl.has_chunk ? ((uint64_t(ctl.chunk) << 32) | ctl.fill) : ~0ull;
@@ -444,8 +479,7 @@ static hipError_t launch_join_kernel(const JoinArgs& A, int key_bytes, int base,
   return hipGetLastError();
 }
 
-hipError_t launch_join(const Layout& L, int key_bytes, int base, int64_t slice, int unique, pj_pair* out,
-                       int64_t capacity, hipStream_t s) {
+static JoinArgs join_args(const Layout& L, int base, int64_t slice, pj_pair* out, int64_t capacity) {
   JoinArgs A{};
   const int last = L.pp.passes == 2;
   A.rt = last ? L.t[0].buf2 : L.t[0].buf1;
@@ -463,6 +497,17 @@ hipError_t launch_join(const Layout& L, int key_bytes, int base, int64_t slice, 
   A.wg_last = L.wg_last;
   A.nr = L.t[0].n;
   A.ns = L.t[1].n;
+  A.desc = reinterpret_cast<const uint4*>(L.desc);
+  A.desc_cap = L.desc_cap;
+  return A;
+}
+
+hipError_t launch_join(const Layout& L, int key_bytes, int base, int64_t slice, int unique, pj_pair* out,
+                       int64_t capacity, hipStream_t s) {
+  JoinArgs A = join_args(L, base, slice, out, capacity);
+  hipLaunchKernelGGL(k_items_expand, dim3(1024), dim3(256), 0, s, A, reinterpret_cast<uint4*>(L.desc), L.desc_cap);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
   return launch_join_kernel(A, key_bytes, base, unique, s);
 }
 
@@ -486,7 +531,8 @@ size_t join_generic_ws_bytes(int B) {
   b += 4 * (size_t(kJoinGrid) + 3) * 8;               // hole/stray arrays
   b += size_t(kJoinGrid) * size_t(kOutChunk) * 8;     // spill
   b += 2 * 4096 * 8;                                  // scan block totals
-  return b + 2048;
+  b += (2 * P + 65536) * 32;                          // work item descriptors
+  return b + 4096;
 }
 
 hipError_t launch_join_generic(const void* rt, const uint32_t* roff, const void* st, const uint32_t* soff,
@@ -517,6 +563,8 @@ hipError_t launch_join_generic(const void* rt, const uint32_t* roff, const void*
   L.stray_pre = reinterpret_cast<uint64_t*>(take((size_t(kJoinGrid) + 3) * 8));
   L.spill = reinterpret_cast<pj_pair*>(take(size_t(kJoinGrid) * size_t(kOutChunk) * 8));
   L.scan_scratch = take(2 * 4096 * 8);
+  L.desc_cap = 2 * P + 65536;
+  L.desc = take(L.desc_cap * 32);
   hipError_t e = launch_join_plan(L, base, slice, s);
   if (e != hipSuccess) return e;
   e = launch_join(L, key_bytes, base, slice, unique, out, capacity, s);

@@ -135,7 +135,7 @@ __global__ __launch_bounds__(kHistBlock) void k_hist(PartTable t0, PartTable t1,
     for (int i = tid; i < P; i += BLOCK) s_bins[i] = 0;
     if (two_pass)
       for (int i = tid; i < F1; i += BLOCK) s_coarse[i] = 0;
-    __syncthreads();
+    lds_barrier();
     for (uint32_t r = 0; r < nrounds; ++r) {
       uint32_t low[4];
       bool valid[4];
@@ -144,7 +144,7 @@ __global__ __launch_bounds__(kHistBlock) void k_hist(PartTable t0, PartTable t1,
       for (int c = 0; c < 4; ++c)
         if (valid[c]) atomicAdd(&s_bins[(low[c] >> pp.shift0) & pmask], 1u);
     }
-    __syncthreads();
+    lds_barrier();
     uint32_t* row = T.mat + size_t(seg) * P;
     for (int i = tid; i < P; i += BLOCK) {
       const uint32_t v = s_bins[i];
@@ -160,7 +160,7 @@ __global__ __launch_bounds__(kHistBlock) void k_hist(PartTable t0, PartTable t1,
     for (int k = 0; k < WPT; ++k) s_bins[tid + BLOCK * k] = 0;
     if (two_pass)
       for (int i = tid; i < F1; i += BLOCK) s_coarse[i] = 0;
-    __syncthreads();
+    lds_barrier();
     constexpr uint32_t kEpochRounds = 15;  // 15 * 4096 = 61440 keys < 65536
     for (uint32_t r0 = 0; r0 < nrounds; r0 += kEpochRounds) {
       const uint32_t r1 = min(nrounds, r0 + kEpochRounds);
@@ -175,7 +175,7 @@ __global__ __launch_bounds__(kHistBlock) void k_hist(PartTable t0, PartTable t1,
             atomicAdd(&s_bins[d >> 1], 1u << ((d & 1u) << 4));
           }
       }
-      __syncthreads();
+      lds_barrier();
 #pragma unroll
       for (int k = 0; k < WPT; ++k) {
         const uint32_t w = s_bins[tid + BLOCK * k];
@@ -183,7 +183,7 @@ __global__ __launch_bounds__(kHistBlock) void k_hist(PartTable t0, PartTable t1,
         acc[2 * k + 1] += w >> 16;
         s_bins[tid + BLOCK * k] = 0;
       }
-      __syncthreads();
+      lds_barrier();
     }
     uint2* row = reinterpret_cast<uint2*>(T.mat + size_t(seg) * P);
 #pragma unroll
@@ -197,7 +197,7 @@ __global__ __launch_bounds__(kHistBlock) void k_hist(PartTable t0, PartTable t1,
     }
   }
   if (two_pass) {
-    __syncthreads();
+    lds_barrier();
     uint32_t* crow = T.matc + size_t(seg) * F1;
     for (int i = tid; i < F1; i += BLOCK) crow[i] = s_coarse[i];
   }
@@ -390,7 +390,7 @@ hipError_t launch_join_plan(const Layout& L, int base, int64_t slice, hipStream_
 // Shared tile scatter: E items per thread already in registers (digit 0xFFFFFFFF
 // marks an empty item).  Ranks by LDS atomics, scans the tile histogram,
 // advances the per-digit global cursors s_cur, stages the tile in LDS in digit
-// order and writes it out in coalesced runs.  Ends with __syncthreads().
+// order and writes it out in coalesced runs.  Ends with an LDS barrier.
 // ---------------------------------------------------------------------------
 template <int KB, int BLOCK, int E>
 __device__ __forceinline__ void scatter_tile(const typename KeyT<KB>::key_t (&key)[E], const int32_t (&pay)[E],
@@ -403,7 +403,7 @@ __device__ __forceinline__ void scatter_tile(const typename KeyT<KB>::key_t (&ke
   uint32_t rk[E];
 #pragma unroll
   for (int e = 0; e < E; ++e) rk[e] = dg[e] != 0xFFFFFFFFu ? atomicAdd(&s_cnt[dg[e]], 1u) : 0u;
-  __syncthreads();
+  lds_barrier();
   // Exclusive scan of the F digit counts: each thread owns DPT consecutive digits.
   const int DPT = F > BLOCK ? F / BLOCK : 1;
   const int d0 = tid * DPT;
@@ -422,18 +422,18 @@ __device__ __forceinline__ void scatter_tile(const typename KeyT<KB>::key_t (&ke
       run += c;
     }
   }
-  __syncthreads();
+  lds_barrier();
 #pragma unroll
   for (int e = 0; e < E; ++e)
     if (dg[e] != 0xFFFFFFFFu) s_stage[s_base[dg[e]] + rk[e]] = KTr::make(key[e], pay[e]);
-  __syncthreads();
+  lds_barrier();
   for (uint32_t j = tid; j < len; j += BLOCK) {
     const typename KTr::tup_t t = s_stage[j];
     const uint32_t d = digit_of(KTr::low32(KTr::key(t)), shift, mask);
     const uint32_t pos = s_delta[d] + j;
     if (pos < n_out) out[pos] = t;  // always true for exact offsets; guards the table bounds
   }
-  __syncthreads();
+  lds_barrier();
 }
 
 // LDS layout of the scatter kernels: stage[TILE] then cnt, base, cur, delta [F] and tmp.
@@ -476,13 +476,8 @@ __global__ __launch_bounds__(BLOCK) void k_pass1(PartTable t0, PartTable t1, Par
   tup_t* out = static_cast<tup_t*>(T.buf1);
   const key_t* keys = static_cast<const key_t*>(T.keys);
 
-  for (uint32_t tile_lo = lo; tile_lo < hi; tile_lo += TILE) {
-    const uint32_t len = min(uint32_t(TILE), hi - tile_lo);
-    for (int d = tid; d < F; d += BLOCK) s_cnt[d] = 0;
-    __syncthreads();
-    key_t key[E];
-    int32_t pay[E];
-    uint32_t dg[E];
+  // Loads the 4-item groups of one tile into registers (keys and payloads).
+  auto load_tile = [&](uint32_t tile_lo, key_t (&key)[E], int32_t (&pay)[E]) {
 #pragma unroll
     for (int e4 = 0; e4 < E / 4; ++e4) {
       const uint32_t i = tile_lo + (uint32_t(e4) * BLOCK + tid) * 4;
@@ -507,21 +502,43 @@ __global__ __launch_bounds__(BLOCK) void k_pass1(PartTable t0, PartTable t1, Par
 #pragma unroll
           for (int c = 0; c < 4; ++c) pay[e4 * 4 + c] = int32_t(i + c);
         }
-#pragma unroll
-        for (int c = 0; c < 4; ++c) dg[e4 * 4 + c] = digit_of(KTr::low32(key[e4 * 4 + c]), shift, mask);
       } else {
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
-          const int e = e4 * 4 + c;
           const bool valid = i + c < hi;
-          key[e] = valid ? __builtin_nontemporal_load(keys + i + c) : key_t(0);
-          pay[e] = haspay ? (valid ? __builtin_nontemporal_load(T.pay + i + c) : 0) : int32_t(i + c);
-          dg[e] = valid ? digit_of(KTr::low32(key[e]), shift, mask) : 0xFFFFFFFFu;
+          key[e4 * 4 + c] = valid ? __builtin_nontemporal_load(keys + i + c) : key_t(0);
+          pay[e4 * 4 + c] = haspay ? (valid ? __builtin_nontemporal_load(T.pay + i + c) : 0) : int32_t(i + c);
         }
       }
     }
+  };
+  key_t key[E];
+  int32_t pay[E];
+  if (lo < hi) load_tile(lo, key, pay);
+  for (uint32_t tile_lo = lo; tile_lo < hi; tile_lo += TILE) {
+    const uint32_t len = min(uint32_t(TILE), hi - tile_lo);
+    // next tile in flight while this one is ranked, staged and written
+    key_t nkey[E];
+    int32_t npay[E];
+    const uint32_t next_lo = tile_lo + TILE;
+    if (next_lo < hi) load_tile(next_lo, nkey, npay);
+    for (int d = tid; d < F; d += BLOCK) s_cnt[d] = 0;
+    lds_barrier();
+    uint32_t dg[E];
+#pragma unroll
+    for (int e4 = 0; e4 < E / 4; ++e4) {
+      const uint32_t i = tile_lo + (uint32_t(e4) * BLOCK + tid) * 4;
+#pragma unroll
+      for (int c = 0; c < 4; ++c)
+        dg[e4 * 4 + c] = i + c < hi ? digit_of(KTr::low32(key[e4 * 4 + c]), shift, mask) : 0xFFFFFFFFu;
+    }
     scatter_tile<KB, BLOCK, E>(key, pay, dg, len, F, shift, mask, s_stage, s_cnt, s_base, s_cur, s_delta, s_tmp,
                                out, T.n);
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      key[e] = nkey[e];
+      pay[e] = npay[e];
+    }
   }
 }
 
@@ -583,10 +600,23 @@ __global__ __launch_bounds__(BLOCK) void k_pass2(PartTable t0, PartTable t1, Par
     const uint32_t in_hi = seg_hi < nseg ? base1 + T.matc[size_t(seg_hi) * F1 + p1] : T.off[q0 + F2];
     if (in_lo >= in_hi || in_hi > T.n) continue;
     for (int d = tid; d < F2; d += BLOCK) s_cur[d] = T.off[q0 + d] + T.mat[size_t(seg_lo) * P + q0 + d];
+    tup_t tv[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const uint32_t i = in_lo + uint32_t(e) * BLOCK + tid;
+      if (i < in_hi) tv[e] = in[i];
+    }
     for (uint32_t tile_lo = in_lo; tile_lo < in_hi; tile_lo += TILE) {
       const uint32_t len = min(uint32_t(TILE), in_hi - tile_lo);
+      tup_t tn[E];
+      const uint32_t next_lo = tile_lo + TILE;
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const uint32_t i = next_lo + uint32_t(e) * BLOCK + tid;
+        if (next_lo < in_hi && i < in_hi) tn[e] = in[i];
+      }
       for (int d = tid; d < F2; d += BLOCK) s_cnt[d] = 0;
-      __syncthreads();
+      lds_barrier();
       key_t key[E];
       int32_t pay[E];
       uint32_t dg[E];
@@ -594,14 +624,14 @@ __global__ __launch_bounds__(BLOCK) void k_pass2(PartTable t0, PartTable t1, Par
       for (int e = 0; e < E; ++e) {
         const uint32_t i = tile_lo + uint32_t(e) * BLOCK + tid;
         const bool valid = i < in_hi;
-        tup_t t;
-        if (valid) t = in[i];
-        key[e] = valid ? KTr::key(t) : key_t(0);
-        pay[e] = valid ? KTr::pay(t) : 0;
+        key[e] = valid ? KTr::key(tv[e]) : key_t(0);
+        pay[e] = valid ? KTr::pay(tv[e]) : 0;
         dg[e] = valid ? digit_of(KTr::low32(key[e]), pp.shift0, mask) : 0xFFFFFFFFu;
       }
       scatter_tile<KB, BLOCK, E>(key, pay, dg, len, F2, pp.shift0, mask, s_stage, s_cnt, s_base, s_cur, s_delta,
                                  s_tmp, out, T.n);
+#pragma unroll
+      for (int e = 0; e < E; ++e) tv[e] = tn[e];
     }
   }
 }
