@@ -15,7 +15,9 @@ constexpr int kScatterBlock = 512;    // partition pass workgroup
 constexpr int kScatterTile4 = 8192;   // tuples staged in LDS per tile, 4-byte keys (64 KB)
 constexpr int kScatterTile8 = 4096;   // 8-byte keys (64 KB)
 constexpr int kJoinBlock = 512;       // join workgroup
-constexpr int kJoinStage = 2048;      // output pairs staged in LDS before a flush
+constexpr int kJoinProbe4 = 9;        // probe tuples per thread and round, 4-byte keys (4608 per round)
+constexpr int kJoinProbe8 = 4;        // 8-byte keys (2048 per round)
+constexpr int kJoinLdsBytes = 81408;  // dynamic LDS per join workgroup: two workgroups share a CU's 160 KB
 constexpr int64_t kOutChunk = 16384;  // output pairs claimed per global allocation
 constexpr int kJoinGrid = 512;        // persistent join workgroups (2 per CU)
 constexpr int kPass2WgPerXcd = 32;    // pass-2 persistent workgroups per XCD and table
@@ -124,6 +126,16 @@ hipError_t launch_join_generic(const void* rt, const uint32_t* roff, const void*
                                int64_t capacity, int64_t* d_count, char* ws, hipStream_t s);
 size_t join_generic_ws_bytes(int B);
 
+// LDS hash table slots of one join item (the stage of one probe round takes
+// the rest of kJoinLdsBytes); the build chunk is at most half of it.
 int join_table_capacity(int key_bytes);
+
+// Slots of the table of a build chunk of c tuples: 8c + 2 (occupancy <= 1/8),
+// at most cap.  c <= cap/2 - 1 keeps at least 2c + 2, so every probe walk
+// ends at an empty slot.
+__host__ __device__ inline uint32_t join_table_slots(uint32_t c, uint32_t cap) {
+  const uint64_t t = 8ull * c + 2ull;
+  return t < cap ? uint32_t(t) : cap;
+}
 
 }  // namespace pj

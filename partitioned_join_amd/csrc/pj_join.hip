@@ -7,16 +7,20 @@
 //                handle_skewed_data reads the block count back to the host,
 //                src/skewed_data_handler.cuh:83, and splits only the build side).
 //   k_join       persistent workgroups; per item: LDS open-addressing table of
-//                2*|chunk| slots (occupancy <= 0.5), MurmurHash3 seed 0, slot =
-//                |(int)h % size| with int_fastdiv, linear probing with LDS CAS
-//                (src/hash_join_kernels.cuh:84-129), then the probe slice walks
-//                the table (:134-181).  Matches are staged in LDS and flushed
-//                into output chunks of kOutChunk pairs that each workgroup claims
-//                with one global atomic (a same-address device atomic costs
-//                ~12 ns on MI355X, so claims stay rare).  The empty-slot
-//                sentinel is a key whose low bits differ from the partition
-//                digit, so every real key value can be joined (the reference
-//                reserves -1, :20,86,106).
+//                8*|chunk| + 2 slots (occupancy <= 1/8), MurmurHash3 seed 0,
+//                slot = |(int)h % size| with int_fastdiv, linear probing with
+//                LDS CAS (src/hash_join_kernels.cuh:84-129), then the probe
+//                slice walks the table (:134-181).  Each slot holds the key and
+//                the build payload together, so a match needs no second read.
+//                Probes of one thread walk interleaved: a step advances every
+//                unfinished probe by one slot, so a wave waits for its longest
+//                chain once per round instead of once per probe.  Matches are
+//                staged in LDS and flushed into output chunks of kOutChunk
+//                pairs that each workgroup claims with one global atomic (a
+//                same-address device atomic costs ~12 ns on MI355X, so claims
+//                stay rare).  The empty-slot sentinel is a key whose low bits
+//                differ from the partition digit, so every real key value can
+//                be joined (the reference reserves -1, :20,86,106).
 //   k_defrag_*   the last chunk of each workgroup is partly filled; pairs that
 //                lie past the total are moved into those holes so the output
 //                is dense in [0, total).
@@ -25,17 +29,24 @@
 
 namespace pj {
 
-int join_table_capacity(int key_bytes) { return key_bytes == 4 ? 7936 : 5120; }
+static int join_probe_per_thread(int key_bytes) { return key_bytes == 4 ? kJoinProbe4 : kJoinProbe8; }
+static int join_stage_pairs(int key_bytes) { return join_probe_per_thread(key_bytes) * kJoinBlock; }
+static int join_slot_bytes(int key_bytes) { return key_bytes == 4 ? 8 : 16; }
 
-static size_t join_lds_bytes(int key_bytes, int table_slots, int stage_pairs) {
-  return size_t(table_slots) * (key_bytes == 4 ? 8 : 12) + size_t(stage_pairs) * 8 + 256;
+int join_table_capacity(int key_bytes) {
+  const int bytes = kJoinLdsBytes - join_stage_pairs(key_bytes) * 8;
+  return (bytes / join_slot_bytes(key_bytes)) & ~1;
+}
+
+static size_t join_lds_bytes(int key_bytes, int table_slots) {
+  return size_t(table_slots) * join_slot_bytes(key_bytes) + size_t(join_stage_pairs(key_bytes)) * 8;
 }
 
 // ---------------------------------------------------------------------------
 // Output staging shared by the join workgroup.
 // ---------------------------------------------------------------------------
 struct StageCtl {
-  uint32_t n;       // staged pairs
+  uint32_t n;       // pairs appended to the stage (may exceed its size: those were not written)
   uint32_t chunk;   // current output chunk
   uint32_t fill;    // pairs written to the current chunk
   uint32_t has_chunk;
@@ -59,18 +70,19 @@ __device__ __forceinline__ void store_pair(const OutSink o, uint64_t v, uint2 pr
   else if (v - o.capacity < o.spill_cap) o.spill[v - o.capacity] = pr;
 }
 
-// Called by all threads after a barrier (ctl->n stable).  Writes the staged
-// pairs to the workgroup's output chunk, claiming a new chunk with one global
-// atomic when the current one fills.  Two barriers; ends with one.
-template <int BLOCK>
-__device__ __forceinline__ void flush_stage(const OutSink o, StageCtl* ctl, const uint2* s_stage) {
+// Writes the first min(staged, STAGE) staged pairs to the workgroup's output
+// chunk, claiming a new chunk with one global atomic when the current one
+// fills.  Called by all threads with the same `staged`, read from ctl->n after
+// a barrier that follows every append; resets ctl->n once every thread is past
+// that read.  Two barriers; ends with one.
+template <int BLOCK, uint32_t STAGE>
+__device__ __forceinline__ void flush_stage(const OutSink o, StageCtl* ctl, const uint2* s_stage, uint32_t staged) {
+  const uint32_t n = staged < STAGE ? staged : STAGE;
   if (threadIdx.x == 0) {
-    const uint32_t n = ctl->n;
     const uint32_t avail = ctl->has_chunk ? uint32_t(kOutChunk) - ctl->fill : 0u;
     const uint32_t a = n < avail ? n : avail;
     ctl->dst0 = uint64_t(ctl->chunk) * uint64_t(kOutChunk) + ctl->fill;
     ctl->a = a;
-    ctl->pad = n;
     if (n > a) {
       const uint32_t c2 = atomicAdd(o.chunk_ctr, 1u);
       ctl->dst1 = uint64_t(c2) * uint64_t(kOutChunk);
@@ -80,11 +92,11 @@ __device__ __forceinline__ void flush_stage(const OutSink o, StageCtl* ctl, cons
     } else {
       ctl->fill += a;
     }
-    ctl->n = 0;  // the next additions come after the closing barrier
   }
   lds_barrier();
-  const uint32_t n = ctl->pad, a = ctl->a;
+  const uint32_t a = ctl->a;
   const uint64_t d0 = ctl->dst0, d1 = ctl->dst1;
+  if (threadIdx.x == 0) ctl->n = 0;  // every thread read n before the barrier above
   for (uint32_t i = threadIdx.x; i < n; i += BLOCK) {
     const uint64_t v = i < a ? d0 + i : d1 + (i - a);
     store_pair(o, v, s_stage[i]);
@@ -98,9 +110,17 @@ __device__ __forceinline__ uint64_t lds_cas(uint64_t* p, uint64_t cmp, uint64_t 
                             (unsigned long long)val));
 }
 
+// Key word(s) and payload word of an LDS table slot (the slot has the tuple layout).
+__device__ __forceinline__ uint32_t* slot_key(uint2* s) { return &s->x; }
+__device__ __forceinline__ uint64_t* slot_key(uint4* s) { return reinterpret_cast<uint64_t*>(s); }
+__device__ __forceinline__ void slot_set_pay(uint2* s, int32_t p) { s->y = uint32_t(p); }
+__device__ __forceinline__ void slot_set_pay(uint4* s, int32_t p) { s->z = uint32_t(p); }
+
+__device__ __forceinline__ uint32_t uniform(uint32_t x) { return __builtin_amdgcn_readfirstlane(x); }
+
 // One work item: partition q, build chunk [rlo, rhi), probe slice [slo, shi);
-// the LDS table of the chunk has fd.d = 2 * (rhi - rlo) + 2 slots.  Stored as
-// two uint4 words: {q, rlo, rhi, slo}, {shi, magic, shift, add_sign}.
+// the LDS table of the chunk has fd.d = join_table_slots(rhi - rlo) slots.
+// Stored as two uint4 words: {q, rlo, rhi, slo}, {shi, magic, shift, add_sign}.
 struct ItemDesc {
   uint32_t q, rlo, rhi, slo, shi, valid;
   FastDiv fd;
@@ -127,7 +147,7 @@ __device__ __forceinline__ ItemDesc decode_item(const JoinArgs& A, uint64_t item
   const uint64_t slo64 = uint64_t(s0) + sl * uint64_t(A.slice);
   d.slo = uint32_t(min(uint64_t(s1), slo64));
   d.shi = uint32_t(min(uint64_t(s1), slo64 + uint64_t(A.slice)));
-  d.fd = fastdiv_make(int32_t(2 * (d.rhi - d.rlo) + 2));
+  d.fd = fastdiv_make(int32_t(join_table_slots(d.rhi - d.rlo, uint32_t(A.table_cap))));
   d.valid = 1;
   return d;
 }
@@ -153,6 +173,7 @@ __device__ __forceinline__ void raw_desc_load(const JoinArgs& A, uint64_t item, 
   b = A.desc[2 * i + 1];
 }
 
+// Decoded descriptor, moved to scalar registers (every field is uniform).
 __device__ __forceinline__ ItemDesc desc_from_raw(const JoinArgs& A, const uint4& a, const uint4& b, uint64_t item,
                                                   uint64_t total) {
   ItemDesc d;
@@ -162,17 +183,29 @@ __device__ __forceinline__ ItemDesc desc_from_raw(const JoinArgs& A, const uint4
     d.fd = FastDiv{2, 0, -1, 0};
     return d;
   }
-  if (item >= A.desc_cap) return decode_item(A, item);  // beyond the expanded descriptors
-  d.q = a.x;
-  d.rlo = a.y;
-  d.rhi = a.z;
-  d.slo = a.w;
-  d.shi = b.x;
-  d.fd.d = int32_t(2 * (d.rhi - d.rlo) + 2);
-  d.fd.magic = int32_t(b.y);
-  d.fd.shift = int32_t(b.z);
-  d.fd.add_sign = int32_t(b.w);
-  d.valid = 1;
+  if (item >= A.desc_cap) {  // beyond the expanded descriptors
+    d = decode_item(A, item);
+  } else {
+    d.q = a.x;
+    d.rlo = a.y;
+    d.rhi = a.z;
+    d.slo = a.w;
+    d.shi = b.x;
+    d.fd.d = int32_t(join_table_slots(d.rhi - d.rlo, uint32_t(A.table_cap)));
+    d.fd.magic = int32_t(b.y);
+    d.fd.shift = int32_t(b.z);
+    d.fd.add_sign = int32_t(b.w);
+    d.valid = 1;
+  }
+  d.q = uniform(d.q);
+  d.rlo = uniform(d.rlo);
+  d.rhi = uniform(d.rhi);
+  d.slo = uniform(d.slo);
+  d.shi = uniform(d.shi);
+  d.fd.d = int32_t(uniform(uint32_t(d.fd.d)));
+  d.fd.magic = int32_t(uniform(uint32_t(d.fd.magic)));
+  d.fd.shift = int32_t(uniform(uint32_t(d.fd.shift)));
+  d.fd.add_sign = int32_t(uniform(uint32_t(d.fd.add_sign)));
   return d;
 }
 
@@ -180,10 +213,12 @@ __device__ __forceinline__ ItemDesc desc_from_raw(const JoinArgs& A, const uint4
 // k_join: software-pipelined persistent join.  While item i is built and
 // probed, the build tuples and first probe round of item i + grid are in
 // flight into registers and the descriptor of item i + 2*grid is loading.
-// Each thread holds up to NB build and R probe tuples; a round of R probes per
-// thread runs as iterations in which every pending probe advances to its next
-// match (one match per probe per iteration) and matches are appended to the
-// LDS stage with one LDS atomic per wave (wave prefix sum).
+// Each thread holds up to NB build and R probe tuples.  A probe round runs
+// steps in which every unfinished probe of the thread reads its next slot; a
+// wave steps until its probes have all reached an empty slot (or, with
+// UNIQUE, their match), appending matches to the LDS stage with one LDS
+// atomic per wave and step.  When the stage is full the wave stops with the
+// unwritten matches pending, the workgroup flushes and the round resumes.
 // ---------------------------------------------------------------------------
 template <int KB, int BLOCK, int NB, int R, bool UNIQUE>
 __global__ __launch_bounds__(BLOCK) void k_join(JoinArgs A) {
@@ -191,16 +226,17 @@ __global__ __launch_bounds__(BLOCK) void k_join(JoinArgs A) {
   using key_t = typename KTr::key_t;
   using tup_t = typename KTr::tup_t;
   constexpr uint32_t ROUND = uint32_t(R) * BLOCK;
+  constexpr uint32_t STAGE = ROUND;
+  constexpr int CNT_BITS = R < 2 ? 1 : (R < 4 ? 2 : (R < 8 ? 3 : (R < 16 ? 4 : 5)));  // bits of a count <= R
+  static_assert(STAGE >= uint32_t(R) * kWave, "one wave's matches of a step must fit an empty stage");
   extern __shared__ __align__(16) uint8_t s_raw[];
-  uint2* s_stage = reinterpret_cast<uint2*>(s_raw);  // ROUND pairs
-  key_t* s_tkeys = reinterpret_cast<key_t*>(s_raw + size_t(ROUND) * 8);
-  int32_t* s_tpays = reinterpret_cast<int32_t*>(s_tkeys + A.table_cap);
+  tup_t* s_table = reinterpret_cast<tup_t*>(s_raw);                                     // table_cap slots
+  uint2* s_stage = reinterpret_cast<uint2*>(s_raw + size_t(A.table_cap) * sizeof(tup_t));  // STAGE pairs
   __shared__ StageCtl ctl;
   __shared__ uint32_t s_any[3];
   uint32_t vote = 0;
 
   const int tid = threadIdx.x;
-  const uint32_t lane = lane_id();
   const tup_t* rt = static_cast<const tup_t*>(A.rt);
   const tup_t* st = static_cast<const tup_t*>(A.st);
   const uint64_t total_items = A.item_scan[A.P];
@@ -243,23 +279,34 @@ __global__ __launch_bounds__(BLOCK) void k_join(JoinArgs A) {
     raw_desc_load(A, item_nn, ra, rb);  // decoded at the end of this item
     const uint32_t tsize = uint32_t(d.fd.d);
     const key_t sentinel = key_t(d.q ^ 1u);
-    for (uint32_t i = tid; i < tsize; i += BLOCK) s_tkeys[i] = sentinel;
+    const tup_t empty = KTr::make(sentinel, 0);
+    for (uint32_t i = tid; i < tsize; i += BLOCK) s_table[i] = empty;
     lds_barrier();
-    // Build.
+    // Build: every thread's tuples claim slots together, one CAS each per step.
+    {
+      uint32_t bslot[NB];
+      uint32_t pend = 0;
 #pragma unroll
-    for (int k = 0; k < NB; ++k) {
-      const uint32_t i = d.rlo + uint32_t(k) * BLOCK + tid;
-      if (i < d.rhi) {
-        const key_t key = KTr::key(bt[k]);
-        uint32_t slot = hash_slot(murmur3(key), d.fd);
-        while (true) {
-          const key_t old = lds_cas(&s_tkeys[slot], sentinel, key);
-          if (old == sentinel) {
-            s_tpays[slot] = KTr::pay(bt[k]);
-            break;
+      for (int k = 0; k < NB; ++k) {
+        const bool v = d.rlo + uint32_t(k) * BLOCK + tid < d.rhi;
+        pend |= v ? (1u << k) : 0u;
+        bslot[k] = v ? hash_slot(murmur3(KTr::key(bt[k])), d.fd) : 0u;
+      }
+      while (pend) {
+        key_t old[NB];
+#pragma unroll
+        for (int k = 0; k < NB; ++k)
+          if (pend & (1u << k)) old[k] = lds_cas(slot_key(&s_table[bslot[k]]), sentinel, KTr::key(bt[k]));
+#pragma unroll
+        for (int k = 0; k < NB; ++k)
+          if (pend & (1u << k)) {
+            if (old[k] == sentinel) {
+              slot_set_pay(&s_table[bslot[k]], KTr::pay(bt[k]));
+              pend &= ~(1u << k);
+            } else {
+              bslot[k] = bslot[k] + 1 == tsize ? 0u : bslot[k] + 1;
+            }
           }
-          slot = (slot + 1 == tsize) ? 0u : slot + 1;
-        }
       }
     }
     lds_barrier();
@@ -269,51 +316,61 @@ __global__ __launch_bounds__(BLOCK) void k_join(JoinArgs A) {
     // need no wait that would also drain the next item's prefetch.
     auto probe_round = [&](const tup_t (&ptr)[R], uint32_t r0) {
       uint32_t slot[R];
-      uint32_t pend = 0;  // bit k: probe k still walking
+      uint32_t walk = 0;  // bit k: probe k has not reached its end
 #pragma unroll
       for (int k = 0; k < R; ++k) {
-        const uint32_t j = r0 + uint32_t(k) * BLOCK + tid;
-        const bool v = j < d.shi;
-        pend |= v ? (1u << k) : 0u;
+        const bool v = r0 + uint32_t(k) * BLOCK + tid < d.shi;
+        walk |= v ? (1u << k) : 0u;
         slot[k] = v ? hash_slot(murmur3(KTr::key(ptr[k])), d.fd) : 0u;
       }
       while (true) {
-        uint32_t has = 0;  // bit k: probe k matched at slot[k] in this iteration
+        while (__ballot(walk != 0)) {
+          key_t e[R];
 #pragma unroll
-        for (int k = 0; k < R; ++k) {
-          if (pend & (1u << k)) {
-            const key_t key = KTr::key(ptr[k]);
-            uint32_t s = slot[k];
-            while (true) {
-              const key_t kk = s_tkeys[s];
-              if (kk == sentinel) { pend &= ~(1u << k); break; }
-              if (kk == key) { has |= 1u << k; break; }
-              s = (s + 1 == tsize) ? 0u : s + 1;
+          for (int k = 0; k < R; ++k)
+            if (walk & (1u << k)) e[k] = *slot_key(&s_table[slot[k]]);
+          uint32_t hit = 0;
+#pragma unroll
+          for (int k = 0; k < R; ++k)
+            if (walk & (1u << k)) {
+              const key_t ek = e[k];
+              if (ek == KTr::key(ptr[k])) hit |= 1u << k;
+              else if (ek == sentinel) walk &= ~(1u << k);
+              else slot[k] = slot[k] + 1 == tsize ? 0u : slot[k] + 1;
             }
-            slot[k] = s;
+          if (__ballot(hit != 0)) {
+            // wave prefix sum of the per-lane match counts from CNT_BITS ballots
+            const uint32_t cnt = __popc(hit);
+            uint32_t excl = 0, wtotal = 0;
+#pragma unroll
+            for (int b = 0; b < CNT_BITS; ++b) {
+              const uint64_t m = __ballot((cnt >> b) & 1u);
+              excl += mbcnt(m) << b;
+              wtotal += uint32_t(__popcll(m)) << b;
+            }
+            uint32_t wbase = 0;
+            if (lane_id() == 0) wbase = atomicAdd(&ctl.n, wtotal);
+            wbase = uniform(wbase);
+            uint32_t pos = wbase + excl;
+#pragma unroll
+            for (int k = 0; k < R; ++k)
+              if (hit & (1u << k)) {
+                if (pos < STAGE) {
+                  s_stage[pos] = make_uint2(uint32_t(KTr::pay(s_table[slot[k]])), uint32_t(KTr::pay(ptr[k])));
+                  if (UNIQUE) walk &= ~(1u << k);
+                  else slot[k] = slot[k] + 1 == tsize ? 0u : slot[k] + 1;
+                }
+                ++pos;  // a match past the stage stays pending at its slot
+              }
+            if (wbase + wtotal > STAGE) break;  // stage full: flush, then resume
           }
         }
-        const uint32_t cnt = __popc(has);
-        // wave prefix sum of the per-lane match counts; one LDS atomic per wave
-        uint32_t incl = cnt;
-#pragma unroll
-        for (int dlt = 1; dlt < kWave; dlt <<= 1) {
-          const uint32_t y = __shfl_up(incl, dlt, kWave);
-          if (lane >= uint32_t(dlt)) incl += y;
-        }
-        const uint32_t wtotal = __shfl(incl, kWave - 1, kWave);
-        uint32_t wbase = 0;
-        if (lane == 0 && wtotal) wbase = atomicAdd(&ctl.n, wtotal);
-        uint32_t pos = __shfl(wbase, 0, kWave) + incl - cnt;
-#pragma unroll
-        for (int k = 0; k < R; ++k)
-          if (has & (1u << k)) {
-            s_stage[pos++] = make_uint2(uint32_t(s_tpays[slot[k]]), uint32_t(KTr::pay(ptr[k])));
-            if (UNIQUE) pend &= ~(1u << k);
-            slot[k] = (slot[k] + 1 == tsize) ? 0u : slot[k] + 1;
-          }
-        const bool any = block_any(pend != 0, s_any, vote++);
-        if (ctl.n > 0) flush_stage<BLOCK>(sink, &ctl, s_stage);
+        // A wave leaves its steps with probes pending only after filling the
+        // stage, so `any` implies staged > 0; flushing on either keeps every
+        // thread on the same barriers.
+        const bool any = block_any(walk != 0, s_any, vote++);
+        const uint32_t staged = ctl.n;
+        if (staged > 0 || any) flush_stage<BLOCK, STAGE>(sink, &ctl, s_stage, staged);
         if (!any) break;
       }
     };
@@ -327,7 +384,9 @@ __global__ __launch_bounds__(BLOCK) void k_join(JoinArgs A) {
       }
       probe_round(pr, r0);
     }
-    lds_barrier();
+    // The last barrier of the probe (block_any or flush) follows every table
+    // read, so the next item may overwrite the table (every item has a
+    // non-empty slice).
     // advance the pipeline
     d = dn;
     dn = desc_from_raw(A, ra, rb, item_nn, total_items);
@@ -460,26 +519,21 @@ static void join_launch_t(const JoinArgs& a, int unique, size_t lds, hipStream_t
 }
 
 static hipError_t launch_join_kernel(const JoinArgs& A, int key_bytes, int base, int unique, hipStream_t s) {
-  const int slots = 2 * base + 2;  // table of a chunk of c tuples has 2c + 2 slots
-  JoinArgs a = A;
-  a.table_cap = slots;
-  // build tuples per thread: NB * kJoinBlock >= base
-  const bool small = base <= 4 * kJoinBlock;
+  const size_t lds = join_lds_bytes(key_bytes, A.table_cap);
+  // build tuples per thread: NB * kJoinBlock >= base (base <= table_cap / 2 - 1)
   if (key_bytes == 4) {
-    constexpr int R = 8;
-    const size_t lds = join_lds_bytes(4, slots, R * kJoinBlock);
-    if (small) join_launch_t<4, 4, R>(a, unique, lds, s);
-    else join_launch_t<4, 8, R>(a, unique, lds, s);
+    static_assert(6 * kJoinBlock >= (kJoinLdsBytes - kJoinProbe4 * kJoinBlock * 8) / 16, "NB = 6 covers the cap");
+    if (base <= 3 * kJoinBlock) join_launch_t<4, 3, kJoinProbe4>(A, unique, lds, s);
+    else join_launch_t<4, 6, kJoinProbe4>(A, unique, lds, s);
   } else {
-    constexpr int R = 4;
-    const size_t lds = join_lds_bytes(8, slots, R * kJoinBlock);
-    if (small) join_launch_t<8, 4, R>(a, unique, lds, s);
-    else join_launch_t<8, 8, R>(a, unique, lds, s);
+    static_assert(4 * kJoinBlock >= (kJoinLdsBytes - kJoinProbe8 * kJoinBlock * 8) / 32, "NB = 4 covers the cap");
+    if (base <= 2 * kJoinBlock) join_launch_t<8, 2, kJoinProbe8>(A, unique, lds, s);
+    else join_launch_t<8, 4, kJoinProbe8>(A, unique, lds, s);
   }
   return hipGetLastError();
 }
 
-static JoinArgs join_args(const Layout& L, int base, int64_t slice, pj_pair* out, int64_t capacity) {
+static JoinArgs join_args(const Layout& L, int key_bytes, int base, int64_t slice, pj_pair* out, int64_t capacity) {
   JoinArgs A{};
   const int last = L.pp.passes == 2;
   A.rt = last ? L.t[0].buf2 : L.t[0].buf1;
@@ -499,12 +553,13 @@ static JoinArgs join_args(const Layout& L, int base, int64_t slice, pj_pair* out
   A.ns = L.t[1].n;
   A.desc = reinterpret_cast<const uint4*>(L.desc);
   A.desc_cap = L.desc_cap;
+  A.table_cap = join_table_capacity(key_bytes);
   return A;
 }
 
 hipError_t launch_join(const Layout& L, int key_bytes, int base, int64_t slice, int unique, pj_pair* out,
                        int64_t capacity, hipStream_t s) {
-  JoinArgs A = join_args(L, base, slice, out, capacity);
+  JoinArgs A = join_args(L, key_bytes, base, slice, out, capacity);
   hipLaunchKernelGGL(k_items_expand, dim3(1024), dim3(256), 0, s, A, reinterpret_cast<uint4*>(L.desc), L.desc_cap);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
