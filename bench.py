@@ -61,6 +61,31 @@ def run_join_steps(pj, torch, bk, pk, plan, steps, warmup):
     return (t1 - t0) * 1000.0 / steps, mean, count
 
 
+def copy_calibration(torch, gib=1, reps=3):
+    """A plain device copy of a known size, run under the same rocprofv3 --pmc
+    pass as the join: its FETCH_SIZE / WRITE_SIZE per launch against the exact
+    bytes moved give the factor by which the counters are scaled."""
+    n = (gib << 30) // 4
+    src = torch.ones(n, dtype=torch.int32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    del src, dst
+    torch.cuda.empty_cache()
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
+    summary (profiles/traffic_64Mx256M.json), or None."""
+    path = os.path.join(ROOT, "profiles", "traffic_64Mx256M.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        rec = json.load(f).get("kernels", {}).get(kernel)
+    return None if rec is None else rec.get("bytes_per_launch")
+
+
 def cpu_baseline(scale_shift=2):
     """Reference inner_join_cpu (oracle/_ref) on the same distribution at
     1/2^scale_shift size; falls back to the oracle restatement if the
@@ -94,6 +119,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-skew", action="store_true", help="skip the Zipf(1.0) comparison")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run a 1 GiB device copy (counter calibration for rocprofv3 --pmc runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the path shards by top radix "
@@ -133,7 +160,7 @@ def main():
                                "probe_idx) pairs", "build_rows": n_r, "probe_rows": n_s, "pairs": count,
                    "plan": plan.as_dict()},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dom,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dom), "kernel": dom,
                      "kernel_ms": kern[dom], "algorithmic_bytes": ab[dom]},
         "phases_ms": ph,
         "whole_job_roofline_frac": ab["total"] / (ms / 1000.0) / 1e9 / HBM_PEAK_GBS,
@@ -141,13 +168,28 @@ def main():
     }
     del bk, pk
     torch.cuda.empty_cache()
+    if args.copy_calibration:
+        copy_calibration(torch)
     if not args.no_skew:
         zb, zp = G.workload_config3(1.0)
         zplan = pj.plan_default(zb.numel(), zp.numel())
         zms, zph, zcount = run_join_steps(pj, torch, zb, zp, zplan, max(3, args.steps // 2), 1)
-        line["zipf_theta1"] = {"workload": "16M unique int32 build x 256M Zipf(1.0) probes", "pairs": zcount,
-                               "plain_tuples_per_s": (zb.numel() + zp.numel()) / (zms / 1000.0),
-                               "plain_ms": zms, "plain_phases_ms": zph, "learned_tuples_per_s": None}
+        rec = {"workload": "16M unique int32 build x 256M Zipf(1.0) probes", "pairs": zcount,
+               "plain_tuples_per_s": (zb.numel() + zp.numel()) / (zms / 1000.0), "plain_ms": zms,
+               "plain_plan": zplan.as_dict(), "plain_phases_ms": zph, "learned_tuples_per_s": None}
+        from partitioned_join_amd import planner
+        w = planner.load_weights()
+        if w is not None:
+            t0 = time.perf_counter()
+            lplan, fallback = planner.plan_learned(zb, zp, w)
+            plan_ms = (time.perf_counter() - t0) * 1000.0
+            lms, lph, lcount = run_join_steps(pj, torch, zb, zp, lplan, max(3, args.steps // 2), 1)
+            assert lcount == zcount, (lcount, zcount)
+            rec.update({"learned_tuples_per_s": (zb.numel() + zp.numel()) / (lms / 1000.0), "learned_ms": lms,
+                        "learned_plan": lplan.as_dict(), "learned_fallback_to_plain": fallback,
+                        "learned_planning_ms": plan_ms, "learned_phases_ms": lph,
+                        "learned_pair_count_equal_plain": lcount == zcount})
+        line["zipf_theta1"] = rec
         del zb, zp
         torch.cuda.empty_cache()
     if not args.no_cpu:

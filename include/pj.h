@@ -142,6 +142,16 @@ pj_status pj_partitioned_join_async(const pj_table* build, const pj_table* probe
 pj_status pj_project_reference(const pj_table* build, const pj_pair* pairs, int64_t n, void* out_xy,
                                hipStream_t stream);
 
+/* Gather map: out_build[i] = build_col[pairs[i].build_idx] and out_probe[i] =
+ * probe_col[pairs[i].probe_idx] for i < n, for columns of 4- or 8-byte
+ * elements; either column may be NULL (not gathered).  Requires pairs made
+ * with row-id payloads.  The reference returns no indices to gather with; this
+ * is the consumer of cuDF's inner_join gather maps
+ * (cudf:include/cudf/join.hpp:95-100, cudf:src/join/join.cu:33-64). */
+pj_status pj_gather(const pj_pair* pairs, int64_t n, const void* build_col, int32_t build_bytes,
+                    const void* probe_col, int32_t probe_bytes, void* out_build, void* out_probe,
+                    hipStream_t stream);
+
 /* ------------------------------------------------------------------------ */
 /* Phase-level entry points (each mirrors one reference host function)        */
 /* ------------------------------------------------------------------------ */
@@ -263,6 +273,53 @@ pj_status pj_gen_row_ids(int32_t* d_pay, int64_t n, hipStream_t stream);
 pj_status pj_profile_enable(int32_t on);
 /* Synchronises on the last recorded events and returns the phase times. */
 pj_status pj_profile_read(pj_phase_times* times);
+
+/* ------------------------------------------------------------------------ */
+/* Learned planner.  No reference counterpart: the reference's plan is fixed  */
+/* by its -n/-N/-f flags (main.cu:100-209).  The planner only chooses a plan; */
+/* joins made with any plan give the same pairs.                             */
+/*  - features: 1030 floats = a 4096-key probe sample's 1024-bin hash        */
+/*    histogram (bins sorted by count, as fractions) + log2|R|/32, log2|S|/32,*/
+/*    log2(|S|/|R|)/8, key_bytes/8, top sampled key frequency, sample        */
+/*    collision probability;                                                */
+/*  - candidates: the 64 most frequent sampled keys (count 0 = no candidate);*/
+/*  - MLP 1030 -> 256 -> 256 -> 81 logits (ReLU): bits of pass 1 (6..12),    */
+/*    bits of pass 2 (6..12), pass count (1..3), 64 heavy-key flags;        */
+/*  - weights: pj_planner_param_count() floats, W1 [1030x256], b1, W2         */
+/*    [256x256], b2, W3 [256x81], b3, row-major.                           */
+/* ------------------------------------------------------------------------ */
+size_t pj_planner_param_count(void);
+/* Largest build chunk of one join work item (LDS table capacity / 2 - 1); a
+ * plan with ceil(f * |R| / P) above it splits every partition's build side. */
+int32_t pj_join_chunk_capacity(int32_t key_bytes);
+size_t pj_planner_scratch_bytes(int32_t batch);
+pj_status pj_planner_sketch(const pj_table* build, const pj_table* probe, float* d_features, uint64_t* d_cand_keys,
+                            uint32_t* d_cand_sample_counts, hipStream_t stream);
+/* Exact occurrences of the candidates in the probe column (d_counts[64]). */
+pj_status pj_planner_candidate_counts(const pj_table* probe, const uint64_t* d_cand_keys,
+                                      const uint32_t* d_cand_sample_counts, uint64_t* d_counts, hipStream_t stream);
+/* logits[batch][81] for features[batch][1030]. */
+pj_status pj_planner_forward(const float* d_weights, const float* d_features, int32_t batch, float* d_logits,
+                             void* d_scratch, size_t scratch_bytes, hipStream_t stream);
+/* One Adam step (beta 0.9/0.999) on the mean loss of a batch: softmax cross-
+ * entropy on the three plan heads (labels[b] = {bits1 - 6, bits2 - 6 or -1,
+ * passes - 1}) plus binary cross-entropy on the heavy flags (heavy[b][j] = 1,
+ * 0, or -1 where there is no candidate); *d_loss receives the mean loss. */
+pj_status pj_planner_train_step(float* d_weights, float* d_adam_m, float* d_adam_v, int32_t step, float lr,
+                                const float* d_features, const int32_t* d_labels, const float* d_heavy,
+                                int32_t batch, float* d_loss, void* d_scratch, size_t scratch_bytes,
+                                hipStream_t stream);
+/* Host only: the plan from one sample's logits.  The predicted heavy set (flag
+ * logit > 0) must equal the exact one (count * 1024 >= |S|) on every candidate
+ * and the plan's build chunks must fit the LDS table; otherwise the plain
+ * plan is returned and *fallback = 1. */
+pj_status pj_plan_decode(const float* logits, const uint32_t* cand_sample_counts, const uint64_t* cand_exact_counts,
+                         const pj_table* build, const pj_table* probe, pj_plan* plan, int32_t* fallback);
+size_t pj_plan_learned_workspace_size(void);
+/* Sketch, forward pass, exact candidate counts, then pj_plan_decode.
+ * Synchronises `stream`. */
+pj_status pj_plan_learned(const pj_table* build, const pj_table* probe, const float* d_weights, pj_plan* plan,
+                          int32_t* fallback, void* workspace, size_t workspace_bytes, hipStream_t stream);
 
 #ifdef __cplusplus
 }

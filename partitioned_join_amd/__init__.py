@@ -179,6 +179,29 @@ def project_reference(build_keys, pairs, stream=None):
     return out
 
 
+def gather(pairs, build_col=None, probe_col=None, stream=None):
+    """Payload columns of the joined rows (cuDF gather map): returns
+    (build_col[pairs[:, 0]], probe_col[pairs[:, 1]]), None for a column not
+    given.  Columns are int32/int64 (or float32/float64) device tensors."""
+    torch = _torch()
+    n = pairs.shape[0]
+
+    def prep(col):
+        if col is None:
+            return None, 4, None
+        if col.element_size() not in (4, 8) or not col.is_contiguous():
+            raise ValueError("gathered columns must be contiguous with 4- or 8-byte elements")
+        return col, col.element_size(), torch.empty(n, dtype=col.dtype, device=pairs.device)
+
+    b, bb, ob = prep(build_col)
+    p, pb, op = prep(probe_col)
+    _lib.check(lib().pj_gather(_ptr(pairs), n, _ptr(b) if b is not None else None, bb,
+                               _ptr(p) if p is not None else None, pb,
+                               _ptr(ob) if ob is not None else None, _ptr(op) if op is not None else None,
+                               _stream(stream)))
+    return ob, op
+
+
 # --------------------------------------------------------------------------- phases
 def combined_histogram_pass(keys, total_bits: int, bits2: int, stream=None):
     """(fine histogram of key & (2^total_bits-1), pass-1 histogram) as int32

@@ -58,6 +58,7 @@ SIGNATURES = {
                                     c_vp]),
     "pj_partitioned_join_async": (c_i32, [P_TABLE, P_TABLE, P_PLAN, c_vp, c_i64, c_vp, c_vp, c_size, c_vp]),
     "pj_project_reference": (c_i32, [P_TABLE, c_vp, c_i64, c_vp, c_vp]),
+    "pj_gather": (c_i32, [c_vp, c_i64, c_vp, c_i32, c_vp, c_i32, c_vp, c_vp, c_vp]),
     "pj_combined_histogram": (c_i32, [P_TABLE, c_i32, c_i32, c_vp, c_vp, c_vp, c_size, c_vp]),
     "pj_histogram_workspace_size": (c_size, [c_i32]),
     "pj_histogram": (c_i32, [P_TABLE, c_i32, c_i32, c_vp, c_vp, c_size, c_vp]),
@@ -86,6 +87,17 @@ SIGNATURES = {
     "pj_gen_row_ids": (c_i32, [c_vp, c_i64, c_vp]),
     "pj_profile_enable": (c_i32, [c_i32]),
     "pj_profile_read": (c_i32, [ctypes.POINTER(PjPhaseTimes)]),
+    "pj_planner_param_count": (c_size, []),
+    "pj_join_chunk_capacity": (c_i32, [c_i32]),
+    "pj_planner_scratch_bytes": (c_size, [c_i32]),
+    "pj_planner_sketch": (c_i32, [P_TABLE, P_TABLE, c_vp, c_vp, c_vp, c_vp]),
+    "pj_planner_candidate_counts": (c_i32, [P_TABLE, c_vp, c_vp, c_vp, c_vp]),
+    "pj_planner_forward": (c_i32, [c_vp, c_vp, c_i32, c_vp, c_vp, c_size, c_vp]),
+    "pj_planner_train_step": (c_i32, [c_vp, c_vp, c_vp, c_i32, ctypes.c_float, c_vp, c_vp, c_vp, c_i32, c_vp, c_vp,
+                                      c_size, c_vp]),
+    "pj_plan_decode": (c_i32, [c_vp, c_vp, c_vp, P_TABLE, P_TABLE, P_PLAN, ctypes.POINTER(c_i32)]),
+    "pj_plan_learned_workspace_size": (c_size, []),
+    "pj_plan_learned": (c_i32, [P_TABLE, P_TABLE, c_vp, P_PLAN, ctypes.POINTER(c_i32), c_vp, c_size, c_vp]),
 }
 
 _LIB = None

@@ -15,6 +15,7 @@
 #include "pj_device.h"
 #include "pj_internal.h"
 #include "pj_misc.h"
+#include "pj_planner.h"
 
 namespace pj {
 
@@ -351,6 +352,22 @@ pj_status pj_project_reference(const pj_table* build, const pj_pair* pairs, int6
   if (n < 0 || (n > 0 && (!pairs || !out_xy))) return fail(PJ_ERR_INVALID_ARG, "bad pairs/out_xy/n");
   if (n == 0) return PJ_OK;
   PJ_HIP(project_reference(build->keys, build->key_bytes, pairs, n, out_xy, stream));
+  return PJ_OK;
+}
+
+pj_status pj_gather(const pj_pair* pairs, int64_t n, const void* build_col, int32_t build_bytes,
+                    const void* probe_col, int32_t probe_bytes, void* out_build, void* out_probe,
+                    hipStream_t stream) {
+  if (n < 0 || (n > 0 && !pairs)) return fail(PJ_ERR_INVALID_ARG, "bad pairs/n");
+  if (build_col && build_bytes != 4 && build_bytes != 8)
+    return fail(PJ_ERR_INVALID_ARG, "build column width must be 4 or 8 (got %d)", build_bytes);
+  if (probe_col && probe_bytes != 4 && probe_bytes != 8)
+    return fail(PJ_ERR_INVALID_ARG, "probe column width must be 4 or 8 (got %d)", probe_bytes);
+  if ((build_col && !out_build) || (probe_col && !out_probe))
+    return fail(PJ_ERR_INVALID_ARG, "a gathered column needs an output buffer");
+  if (n == 0 || (!build_col && !probe_col)) return PJ_OK;
+  PJ_HIP(gather_columns(pairs, n, build_col, build_col ? build_bytes : 4, probe_col, probe_col ? probe_bytes : 4,
+                        out_build, out_probe, stream));
   return PJ_OK;
 }
 
@@ -708,6 +725,151 @@ pj_status pj_profile_read(pj_phase_times* t) {
   PJ_HIP(hipEventElapsedTime(&t->total_ms, g_prof.ev[0], g_prof.ev[7]));
   t->valid = 1;
   return PJ_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Learned planner
+// ---------------------------------------------------------------------------
+size_t pj_planner_param_count(void) { return kPlanParams; }
+
+int32_t pj_join_chunk_capacity(int32_t key_bytes) {
+  return key_bytes == 4 || key_bytes == 8 ? join_table_capacity(key_bytes) / 2 - 1 : 0;
+}
+
+size_t pj_planner_scratch_bytes(int32_t batch) {
+  return batch < 1 ? 0 : planner_scratch_floats(batch) * sizeof(float);
+}
+
+pj_status pj_planner_sketch(const pj_table* build, const pj_table* probe, float* d_features, uint64_t* d_cand_keys,
+                            uint32_t* d_cand_sample_counts, hipStream_t stream) {
+  pj_status st = check_pair(build, probe);
+  if (st != PJ_OK) return st;
+  if (!d_features || !d_cand_keys || !d_cand_sample_counts) return fail(PJ_ERR_INVALID_ARG, "NULL output");
+  PJ_HIP(planner_sketch(probe->keys, probe->n, probe->key_bytes, build->n, d_features, d_cand_keys,
+                        d_cand_sample_counts, stream));
+  return PJ_OK;
+}
+
+pj_status pj_planner_candidate_counts(const pj_table* probe, const uint64_t* d_cand_keys,
+                                      const uint32_t* d_cand_sample_counts, uint64_t* d_counts, hipStream_t stream) {
+  pj_status st = check_table(probe, "probe");
+  if (st != PJ_OK) return st;
+  if (!d_cand_keys || !d_cand_sample_counts || !d_counts) return fail(PJ_ERR_INVALID_ARG, "NULL argument");
+  PJ_HIP(planner_count_candidates(probe->keys, probe->n, probe->key_bytes, d_cand_keys, d_cand_sample_counts,
+                                  reinterpret_cast<unsigned long long*>(d_counts), stream));
+  return PJ_OK;
+}
+
+pj_status pj_planner_forward(const float* d_weights, const float* d_features, int32_t batch, float* d_logits,
+                             void* d_scratch, size_t scratch_bytes, hipStream_t stream) {
+  if (!d_weights || !d_features || !d_logits || !d_scratch || batch < 1)
+    return fail(PJ_ERR_INVALID_ARG, "bad planner forward arguments");
+  if (scratch_bytes < pj_planner_scratch_bytes(batch))
+    return fail(PJ_ERR_WORKSPACE, "planner scratch needs %zu bytes", pj_planner_scratch_bytes(batch));
+  PJ_HIP(planner_forward(d_weights, d_features, batch, d_logits, static_cast<float*>(d_scratch), stream));
+  return PJ_OK;
+}
+
+pj_status pj_planner_train_step(float* d_weights, float* d_adam_m, float* d_adam_v, int32_t step, float lr,
+                                const float* d_features, const int32_t* d_labels, const float* d_heavy,
+                                int32_t batch, float* d_loss, void* d_scratch, size_t scratch_bytes,
+                                hipStream_t stream) {
+  if (!d_weights || !d_adam_m || !d_adam_v || !d_features || !d_labels || !d_heavy || !d_loss || !d_scratch ||
+      batch < 1 || step < 1 || !(lr > 0.f))
+    return fail(PJ_ERR_INVALID_ARG, "bad planner training arguments");
+  if (scratch_bytes < pj_planner_scratch_bytes(batch))
+    return fail(PJ_ERR_WORKSPACE, "planner scratch needs %zu bytes", pj_planner_scratch_bytes(batch));
+  PJ_HIP(planner_train_step(d_weights, d_adam_m, d_adam_v, step, lr, d_features, d_labels, d_heavy, batch, d_loss,
+                            static_cast<float*>(d_scratch), stream));
+  return PJ_OK;
+}
+
+size_t pj_plan_learned_workspace_size(void) {
+  Carver c{nullptr};
+  c.take(kPlanFeat * sizeof(float));
+  c.take(kPlanCand * 8);
+  c.take(kPlanCand * 4);
+  c.take(kPlanCand * 8);
+  c.take(kPlanOut * sizeof(float));
+  c.take(planner_scratch_floats(1) * sizeof(float));
+  return c.off;
+}
+
+pj_status pj_plan_decode(const float* logits, const uint32_t* cand_sample_counts, const uint64_t* cand_exact_counts,
+                         const pj_table* build, const pj_table* probe, pj_plan* plan, int32_t* fallback) {
+  pj_status st = check_pair(build, probe, false);
+  if (st != PJ_OK) return st;
+  if (!logits || !cand_sample_counts || !cand_exact_counts || !plan) return fail(PJ_ERR_INVALID_ARG, "NULL argument");
+  auto argmax = [&](int off, int n) {
+    int best = 0;
+    for (int c = 1; c < n; ++c)
+      if (logits[off + c] > logits[off + best]) best = c;
+    return best;
+  };
+  // Heavy key: at least 1/1024 of the probe rows.  The predicted set must
+  // equal the exact one on every valid candidate, else the plain plan runs.
+  bool agree = true;
+  const int hoff = 2 * kPlanBitsCls + kPlanPassCls;
+  for (int j = 0; j < kPlanCand; ++j) {
+    if (cand_sample_counts[j] == 0) continue;
+    const bool pred = logits[hoff + j] > 0.f;
+    const bool exact = probe->n > 0 && cand_exact_counts[j] * 1024ull >= uint64_t(probe->n);
+    agree = agree && pred == exact;
+  }
+  pj_plan learned;
+  plain_plan(build->n, build->key_bytes, &learned);
+  const int passes = 1 + argmax(2 * kPlanBitsCls, kPlanPassCls);
+  const int b1 = 6 + argmax(0, kPlanBitsCls), b2 = 6 + argmax(kPlanBitsCls, kPlanBitsCls);
+  if (passes == 1) {
+    learned.passes = 1;
+    learned.bits1 = b1;
+    learned.bits2 = 0;
+  } else {  // three passes are not implemented: two passes with the predicted bits
+    learned.passes = 2;
+    learned.bits1 = b1;
+    learned.bits2 = b1 + b2 > kMaxTotalBits ? kMaxTotalBits - b1 : b2;
+  }
+  learned.source = 1;
+  // a plan whose build chunks would not fit the LDS table is never taken
+  const int cap = join_table_capacity(build->key_bytes) / 2 - 1;
+  const double per = learned.max_partition_size_factor * double(build->n) /
+                     double(int64_t(1) << (learned.bits1 + learned.bits2));
+  const bool fits = per <= double(cap);
+  if (agree && fits && check_plan(&learned) == PJ_OK) {
+    *plan = learned;
+    if (fallback) *fallback = 0;
+  } else {
+    plain_plan(build->n, build->key_bytes, plan);
+    if (fallback) *fallback = 1;
+  }
+  return PJ_OK;
+}
+
+pj_status pj_plan_learned(const pj_table* build, const pj_table* probe, const float* d_weights, pj_plan* plan,
+                          int32_t* fallback, void* workspace, size_t workspace_bytes, hipStream_t stream) {
+  pj_status st = check_pair(build, probe);
+  if (st != PJ_OK) return st;
+  if (!d_weights || !plan || !workspace) return fail(PJ_ERR_INVALID_ARG, "NULL argument");
+  if (workspace_bytes < pj_plan_learned_workspace_size())
+    return fail(PJ_ERR_WORKSPACE, "learned planning needs %zu bytes", pj_plan_learned_workspace_size());
+  Carver c{static_cast<char*>(workspace)};
+  float* feat = static_cast<float*>(c.take(kPlanFeat * sizeof(float)));
+  uint64_t* cand = static_cast<uint64_t*>(c.take(kPlanCand * 8));
+  uint32_t* cand_n = static_cast<uint32_t*>(c.take(kPlanCand * 4));
+  unsigned long long* exact = static_cast<unsigned long long*>(c.take(kPlanCand * 8));
+  float* logits = static_cast<float*>(c.take(kPlanOut * sizeof(float)));
+  float* scratch = static_cast<float*>(c.take(planner_scratch_floats(1) * sizeof(float)));
+  PJ_HIP(planner_sketch(probe->keys, probe->n, probe->key_bytes, build->n, feat, cand, cand_n, stream));
+  PJ_HIP(planner_forward(d_weights, feat, 1, logits, scratch, stream));
+  PJ_HIP(planner_count_candidates(probe->keys, probe->n, probe->key_bytes, cand, cand_n, exact, stream));
+  float h_logits[kPlanOut];
+  uint32_t h_cand_n[kPlanCand];
+  uint64_t h_exact[kPlanCand];
+  PJ_HIP(hipMemcpyAsync(h_logits, logits, sizeof(h_logits), hipMemcpyDeviceToHost, stream));
+  PJ_HIP(hipMemcpyAsync(h_cand_n, cand_n, sizeof(h_cand_n), hipMemcpyDeviceToHost, stream));
+  PJ_HIP(hipMemcpyAsync(h_exact, exact, sizeof(h_exact), hipMemcpyDeviceToHost, stream));
+  PJ_HIP(hipStreamSynchronize(stream));
+  return pj_plan_decode(h_logits, h_cand_n, h_exact, build, probe, plan, fallback);
 }
 
 }  // extern "C"

@@ -50,8 +50,8 @@ struct StageCtl {
   uint32_t chunk;   // current output chunk
   uint32_t fill;    // pairs written to the current chunk
   uint32_t has_chunk;
-  uint32_t a;       // pairs of this flush that go to the current chunk
-  uint32_t pad;
+  uint32_t a;       // pairs of the planned copy that go to the current chunk
+  uint32_t copy_n;  // pairs of the planned copy
   uint64_t dst0, dst1;
 };
 
@@ -70,37 +70,50 @@ __device__ __forceinline__ void store_pair(const OutSink o, uint64_t v, uint2 pr
   else if (v - o.capacity < o.spill_cap) o.spill[v - o.capacity] = pr;
 }
 
-// Writes the first min(staged, STAGE) staged pairs to the workgroup's output
-// chunk, claiming a new chunk with one global atomic when the current one
-// fills.  Called by all threads with the same `staged`, read from ctl->n after
-// a barrier that follows every append; resets ctl->n once every thread is past
-// that read.  Two barriers; ends with one.
-template <int BLOCK, uint32_t STAGE>
-__device__ __forceinline__ void flush_stage(const OutSink o, StageCtl* ctl, const uint2* s_stage, uint32_t staged) {
+// Thread 0 only: plans the copy of the first min(staged, STAGE) staged pairs
+// to the workgroup's output chunk, claiming a new chunk with one global atomic
+// when the current one fills.  `staged` is ctl->n read after a barrier that
+// follows every append; copy_stage() runs after the next barrier.
+template <uint32_t STAGE>
+__device__ __forceinline__ void plan_copy(const OutSink o, StageCtl* ctl, uint32_t staged) {
   const uint32_t n = staged < STAGE ? staged : STAGE;
-  if (threadIdx.x == 0) {
-    const uint32_t avail = ctl->has_chunk ? uint32_t(kOutChunk) - ctl->fill : 0u;
-    const uint32_t a = n < avail ? n : avail;
-    ctl->dst0 = uint64_t(ctl->chunk) * uint64_t(kOutChunk) + ctl->fill;
-    ctl->a = a;
-    if (n > a) {
-      const uint32_t c2 = atomicAdd(o.chunk_ctr, 1u);
-      ctl->dst1 = uint64_t(c2) * uint64_t(kOutChunk);
-      ctl->chunk = c2;
-      ctl->fill = n - a;
-      ctl->has_chunk = 1;
-    } else {
-      ctl->fill += a;
-    }
+  const uint32_t avail = ctl->has_chunk ? uint32_t(kOutChunk) - ctl->fill : 0u;
+  const uint32_t a = n < avail ? n : avail;
+  ctl->dst0 = uint64_t(ctl->chunk) * uint64_t(kOutChunk) + ctl->fill;
+  ctl->a = a;
+  ctl->copy_n = n;
+  if (n > a) {
+    const uint32_t c2 = atomicAdd(o.chunk_ctr, 1u);
+    ctl->dst1 = uint64_t(c2) * uint64_t(kOutChunk);
+    ctl->chunk = c2;
+    ctl->fill = n - a;
+    ctl->has_chunk = 1;
+  } else {
+    ctl->fill += a;
   }
-  lds_barrier();
-  const uint32_t a = ctl->a;
+}
+
+// All threads, after the barrier that follows plan_copy(): write the planned
+// pairs.  Thread 0 resets the stage count: every read of it came before that
+// barrier, and appends resume only after the next one.
+template <int BLOCK>
+__device__ __forceinline__ void copy_stage(const OutSink o, StageCtl* ctl, const uint2* s_stage) {
+  const uint32_t n = ctl->copy_n, a = ctl->a;
   const uint64_t d0 = ctl->dst0, d1 = ctl->dst1;
-  if (threadIdx.x == 0) ctl->n = 0;  // every thread read n before the barrier above
+  if (threadIdx.x == 0) ctl->n = 0;
   for (uint32_t i = threadIdx.x; i < n; i += BLOCK) {
     const uint64_t v = i < a ? d0 + i : d1 + (i - a);
     store_pair(o, v, s_stage[i]);
   }
+}
+
+// Immediate flush: plan, barrier, copy, barrier.  Called by all threads with
+// the same `staged`; ends with a barrier.
+template <int BLOCK, uint32_t STAGE>
+__device__ __forceinline__ void flush_stage(const OutSink o, StageCtl* ctl, const uint2* s_stage, uint32_t staged) {
+  if (threadIdx.x == 0) plan_copy<STAGE>(o, ctl, staged);
+  lds_barrier();
+  copy_stage<BLOCK>(o, ctl, s_stage);
   lds_barrier();
 }
 
@@ -247,6 +260,7 @@ __global__ __launch_bounds__(BLOCK) void k_join(JoinArgs A) {
     ctl.fill = 0;
     ctl.chunk = 0;
     ctl.has_chunk = 0;
+    ctl.copy_n = 0;
     s_any[0] = s_any[1] = s_any[2] = 0;
   }
   lds_barrier();
@@ -282,7 +296,15 @@ __global__ __launch_bounds__(BLOCK) void k_join(JoinArgs A) {
     const tup_t empty = KTr::make(sentinel, 0);
     for (uint32_t i = tid; i < tsize; i += BLOCK) s_table[i] = empty;
     lds_barrier();
+    // The previous item's matches go out now, after this item's prefetch was
+    // issued: the in-order memory counter that the next wait for the prefetch
+    // uses then finds these stores long complete.
+    copy_stage<BLOCK>(sink, &ctl, s_stage);
     // Build: every thread's tuples claim slots together, one CAS each per step.
+    // A CAS that finds the tuple's own key reveals a duplicate build key in
+    // the chunk (two equal keys start at the same slot, so the later one
+    // always meets the earlier one or loses the slot to it).
+    bool dup = false;
     {
       uint32_t bslot[NB];
       uint32_t pend = 0;
@@ -304,17 +326,74 @@ __global__ __launch_bounds__(BLOCK) void k_join(JoinArgs A) {
               slot_set_pay(&s_table[bslot[k]], KTr::pay(bt[k]));
               pend &= ~(1u << k);
             } else {
+              dup |= old[k] == KTr::key(bt[k]);
               bslot[k] = bslot[k] + 1 == tsize ? 0u : bslot[k] + 1;
             }
           }
       }
     }
-    lds_barrier();
-    // Probe rounds of R tuples per thread.  The first round is already in
-    // registers; later rounds (slices longer than one round) load their own
-    // registers, so the first round's are never reloaded inside this item and
-    // need no wait that would also drain the next item's prefetch.
-    auto probe_round = [&](const tup_t (&ptr)[R], uint32_t r0) {
+    // Without duplicate keys in the chunk a probe has at most one match.  The
+    // vote is also the barrier between the build and the probes, so it runs
+    // whether or not the plan already promises unique keys.
+    const bool chunk_has_dup = block_any(dup, s_any, vote++);
+    const bool first_match_only = UNIQUE || !chunk_has_dup;
+
+    // A probe round of R tuples per thread with at most one match per probe:
+    // walk to the match or an empty slot, then append all matches at once
+    // (the stage is empty and holds R per thread).  Ends with a barrier.
+    auto round_single = [&](const tup_t (&ptr)[R], uint32_t r0) {
+      uint32_t slot[R];
+      uint32_t walk = 0, found = 0;
+#pragma unroll
+      for (int k = 0; k < R; ++k) {
+        const bool v = r0 + uint32_t(k) * BLOCK + tid < d.shi;
+        walk |= v ? (1u << k) : 0u;
+        slot[k] = v ? hash_slot(murmur3(KTr::key(ptr[k])), d.fd) : 0u;
+      }
+      while (__ballot(walk != 0)) {
+        key_t e[R];
+#pragma unroll
+        for (int k = 0; k < R; ++k)
+          if (walk & (1u << k)) e[k] = *slot_key(&s_table[slot[k]]);
+#pragma unroll
+        for (int k = 0; k < R; ++k)
+          if (walk & (1u << k)) {
+            if (e[k] == KTr::key(ptr[k])) {
+              found |= 1u << k;
+              walk &= ~(1u << k);
+            } else if (e[k] == sentinel) {
+              walk &= ~(1u << k);
+            } else {
+              slot[k] = slot[k] + 1 == tsize ? 0u : slot[k] + 1;
+            }
+          }
+      }
+      if (__ballot(found != 0)) {
+        const uint32_t cnt = __popc(found);
+        uint32_t excl = 0, wtotal = 0;
+#pragma unroll
+        for (int b = 0; b < CNT_BITS; ++b) {
+          const uint64_t m = __ballot((cnt >> b) & 1u);
+          excl += mbcnt(m) << b;
+          wtotal += uint32_t(__popcll(m)) << b;
+        }
+        uint32_t wbase = 0;
+        if (lane_id() == 0) wbase = atomicAdd(&ctl.n, wtotal);
+        uint32_t pos = uniform(wbase) + excl;
+#pragma unroll
+        for (int k = 0; k < R; ++k)
+          if ((found & (1u << k)) && pos < STAGE)
+            s_stage[pos++] = make_uint2(uint32_t(KTr::pay(s_table[slot[k]])), uint32_t(KTr::pay(ptr[k])));
+      }
+      lds_barrier();
+    };
+
+    // A probe round with any number of matches per probe: steps advance every
+    // unfinished probe to its next match or an empty slot, appending matches
+    // with one LDS atomic per wave and step.  When the stage is full the wave
+    // stops with the unwritten matches pending, the workgroup flushes and the
+    // round resumes.  Ends with a barrier; the last matches stay staged.
+    auto round_multi = [&](const tup_t (&ptr)[R], uint32_t r0) {
       uint32_t slot[R];
       uint32_t walk = 0;  // bit k: probe k has not reached its end
 #pragma unroll
@@ -333,9 +412,8 @@ __global__ __launch_bounds__(BLOCK) void k_join(JoinArgs A) {
 #pragma unroll
           for (int k = 0; k < R; ++k)
             if (walk & (1u << k)) {
-              const key_t ek = e[k];
-              if (ek == KTr::key(ptr[k])) hit |= 1u << k;
-              else if (ek == sentinel) walk &= ~(1u << k);
+              if (e[k] == KTr::key(ptr[k])) hit |= 1u << k;
+              else if (e[k] == sentinel) walk &= ~(1u << k);
               else slot[k] = slot[k] + 1 == tsize ? 0u : slot[k] + 1;
             }
           if (__ballot(hit != 0)) {
@@ -357,8 +435,7 @@ __global__ __launch_bounds__(BLOCK) void k_join(JoinArgs A) {
               if (hit & (1u << k)) {
                 if (pos < STAGE) {
                   s_stage[pos] = make_uint2(uint32_t(KTr::pay(s_table[slot[k]])), uint32_t(KTr::pay(ptr[k])));
-                  if (UNIQUE) walk &= ~(1u << k);
-                  else slot[k] = slot[k] + 1 == tsize ? 0u : slot[k] + 1;
+                  slot[k] = slot[k] + 1 == tsize ? 0u : slot[k] + 1;
                 }
                 ++pos;  // a match past the stage stays pending at its slot
               }
@@ -366,27 +443,33 @@ __global__ __launch_bounds__(BLOCK) void k_join(JoinArgs A) {
           }
         }
         // A wave leaves its steps with probes pending only after filling the
-        // stage, so `any` implies staged > 0; flushing on either keeps every
-        // thread on the same barriers.
+        // stage, so `any` implies staged > 0.
         const bool any = block_any(walk != 0, s_any, vote++);
-        const uint32_t staged = ctl.n;
-        if (staged > 0 || any) flush_stage<BLOCK, STAGE>(sink, &ctl, s_stage, staged);
         if (!any) break;
+        flush_stage<BLOCK, STAGE>(sink, &ctl, s_stage, ctl.n);
       }
     };
-    if (d.slo < d.shi) probe_round(pt, d.slo);
+
+    auto round = [&](const tup_t (&ptr)[R], uint32_t r0) {
+      if (first_match_only) round_single(ptr, r0);
+      else round_multi(ptr, r0);
+    };
+    round(pt, d.slo);
     for (uint32_t r0 = d.slo + ROUND; r0 < d.shi; r0 += ROUND) {
+      // another round of this item: empty the stage first
+      flush_stage<BLOCK, STAGE>(sink, &ctl, s_stage, ctl.n);
       tup_t pr[R];
 #pragma unroll
       for (int k = 0; k < R; ++k) {
         const uint32_t j = r0 + uint32_t(k) * BLOCK + tid;
         if (j < d.shi) pr[k] = st[j];
       }
-      probe_round(pr, r0);
+      round(pr, r0);
     }
-    // The last barrier of the probe (block_any or flush) follows every table
-    // read, so the next item may overwrite the table (every item has a
-    // non-empty slice).
+    // The round's closing barrier follows every table read and stage append:
+    // the next item may overwrite the table, and the stage is copied out after
+    // the next item's first barrier.
+    if (tid == 0) plan_copy<STAGE>(sink, &ctl, ctl.n);
     // advance the pipeline
     d = dn;
     dn = desc_from_raw(A, ra, rb, item_nn, total_items);
@@ -396,6 +479,8 @@ __global__ __launch_bounds__(BLOCK) void k_join(JoinArgs A) {
     for (int k = 0; k < R; ++k) pt[k] = pn[k];
     item += gridDim.x;
   }
+  lds_barrier();
+  copy_stage<BLOCK>(sink, &ctl, s_stage);
   if (tid == 0)
     A.wg_last[blockIdx.x] = ctl.has_chunk ? ((uint64_t(ctl.chunk) << 32) | ctl.fill) : ~0ull;
 }

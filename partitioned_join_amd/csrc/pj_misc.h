@@ -25,6 +25,8 @@ hipError_t hash_keys(const void* keys, int64_t n, int key_bytes, uint32_t* out, 
 hipError_t fastdiv_run(const int32_t* nv, int64_t count, int32_t d, int32_t* q, int32_t* r, hipStream_t s);
 hipError_t project_reference(const void* keys, int key_bytes, const pj_pair* pairs, int64_t n, void* out,
                              hipStream_t s);
+hipError_t gather_columns(const pj_pair* pairs, int64_t n, const void* bcol, int bbytes, const void* pcol,
+                          int pbytes, void* bout, void* pout, hipStream_t s);
 hipError_t digit_hist(const void* keys, int64_t n, int key_bytes, int stride, int shift, int bits, uint32_t* hist,
                       hipStream_t s);
 size_t scan_ws_bytes(int64_t n);

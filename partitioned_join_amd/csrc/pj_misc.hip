@@ -176,6 +176,53 @@ hipError_t project_reference(const void* keys, int key_bytes, const pj_pair* pai
 }
 
 // ---------------------------------------------------------------------------
+// Gather of payload columns through the join's pairs (cuDF's gather map):
+// out_build[i] = build_col[pairs[i].build_idx], out_probe[i] =
+// probe_col[pairs[i].probe_idx].  Each thread handles 4 pairs (two 16-byte
+// pair loads); the column reads are random and the writes coalesced.
+// ---------------------------------------------------------------------------
+template <typename TB, typename TP>
+__global__ __launch_bounds__(256) void k_gather(const uint2* pairs, int64_t n, const TB* bcol, const TP* pcol,
+                                                TB* bout, TP* pout) {
+  const int64_t groups = (n + 3) / 4;
+  PJ_GRID_STRIDE(g, groups) {
+    const int64_t i0 = g * 4;
+    uint2 pr[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) pr[c] = i0 + c < n ? pairs[i0 + c] : make_uint2(0u, 0u);
+    TB vb[4];
+    TP vp[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      if (bcol && i0 + c < n) vb[c] = bcol[int32_t(pr[c].x)];
+      if (pcol && i0 + c < n) vp[c] = pcol[int32_t(pr[c].y)];
+    }
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      if (bcol && i0 + c < n) bout[i0 + c] = vb[c];
+      if (pcol && i0 + c < n) pout[i0 + c] = vp[c];
+    }
+  }
+}
+
+template <typename TB, typename TP>
+static hipError_t gather_t(const pj_pair* pairs, int64_t n, const void* bcol, const void* pcol, void* bout,
+                           void* pout, hipStream_t s) {
+  hipLaunchKernelGGL((k_gather<TB, TP>), dim3(grid_for((n + 3) / 4, 256)), dim3(256), 0, s,
+                     reinterpret_cast<const uint2*>(pairs), n, static_cast<const TB*>(bcol),
+                     static_cast<const TP*>(pcol), static_cast<TB*>(bout), static_cast<TP*>(pout));
+  return hipGetLastError();
+}
+
+hipError_t gather_columns(const pj_pair* pairs, int64_t n, const void* bcol, int bbytes, const void* pcol,
+                          int pbytes, void* bout, void* pout, hipStream_t s) {
+  if (bbytes == 4 && pbytes == 4) return gather_t<int32_t, int32_t>(pairs, n, bcol, pcol, bout, pout, s);
+  if (bbytes == 4 && pbytes == 8) return gather_t<int32_t, int64_t>(pairs, n, bcol, pcol, bout, pout, s);
+  if (bbytes == 8 && pbytes == 4) return gather_t<int64_t, int32_t>(pairs, n, bcol, pcol, bout, pout, s);
+  return gather_t<int64_t, int64_t>(pairs, n, bcol, pcol, bout, pout, s);
+}
+
+// ---------------------------------------------------------------------------
 // Digit histogram (phase API): LDS bins, one global atomic per non-empty bin
 // per workgroup.  d_hist must be zeroed.  tuple_stride: elements between keys
 // (1 for SoA keys, 2 / 2 for 4-byte / 8-byte-key pj tuples).

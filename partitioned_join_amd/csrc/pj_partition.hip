@@ -24,6 +24,8 @@
 //              assignment, src/skewed_data_handler.cuh:58-91).
 // Segment s of a table is processed on XCD s / (nseg/8) by both the histogram
 // and pass 1 (workgroup b runs on XCD b % 8).
+#include <type_traits>
+
 #include "pj_device.h"
 #include "pj_internal.h"
 
@@ -114,6 +116,11 @@ __device__ __forceinline__ void load4_low(const void* keys, uint32_t i, uint32_t
 template <int KB, bool PACKED, bool VEC>
 __global__ __launch_bounds__(kHistBlock) void k_hist(PartTable t0, PartTable t1, PartParams pp) {
   constexpr int BLOCK = kHistBlock;
+  // Rounds of 4 keys per thread loaded together before any is counted, so
+  // each wave keeps BATCH loads in flight (one round at a time leaves the
+  // loop latency-bound at a fraction of HBM bandwidth).
+  constexpr uint32_t BATCH = 5;
+  constexpr uint32_t kEpochBatches = 3;  // packed bins: 3 * 5 * 4096 = 61440 keys < 65536 per epoch
   const PartTable T = blockIdx.y ? t1 : t0;
   const uint32_t b = blockIdx.x;
   if (b >= T.nseg) return;
@@ -130,20 +137,50 @@ __global__ __launch_bounds__(kHistBlock) void k_hist(PartTable t0, PartTable t1,
   uint32_t* s_coarse = s_mem + (PACKED ? P / 2 : P);
   const int tid = threadIdx.x;
   const uint32_t nrounds = (hi - lo + 4 * BLOCK - 1) / (4 * BLOCK);
+  const uint32_t nfull = VEC ? (hi - lo) / (4 * BLOCK * BATCH) : 0;  // batches without a bounds check
+
+  auto count = [&](uint32_t low) {
+    const uint32_t d = (low >> pp.shift0) & pmask;
+    if (PACKED) atomicAdd(&s_bins[d >> 1], 1u << ((d & 1u) << 4));
+    else atomicAdd(&s_bins[d], 1u);
+  };
+  // Batch f: rounds f * BATCH .. f * BATCH + BATCH - 1, all in range.
+  auto count_batch = [&](uint32_t f) {
+    uint32_t low[BATCH][4];
+#pragma unroll
+    for (uint32_t u = 0; u < BATCH; ++u) {
+      const uint32_t i = lo + ((f * BATCH + u) * BLOCK + tid) * 4;
+      if (KB == 4) {
+        const u32x4 v = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(static_cast<const uint32_t*>(T.keys) + i));
+        low[u][0] = v.x; low[u][1] = v.y; low[u][2] = v.z; low[u][3] = v.w;
+      } else {
+        const uint64_t* k = static_cast<const uint64_t*>(T.keys) + i;
+        const u32x4 a = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(k));
+        const u32x4 c = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(k + 2));
+        low[u][0] = a.x; low[u][1] = a.z; low[u][2] = c.x; low[u][3] = c.z;
+      }
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < BATCH; ++u)
+#pragma unroll
+      for (int c = 0; c < 4; ++c) count(low[u][c]);
+  };
+  auto count_round = [&](uint32_t r) {
+    uint32_t low[4];
+    bool valid[4];
+    load4_low<KB, VEC>(T.keys, lo + (r * BLOCK + tid) * 4, hi, low, valid);
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+      if (valid[c]) count(low[c]);
+  };
 
   if (!PACKED) {
     for (int i = tid; i < P; i += BLOCK) s_bins[i] = 0;
     if (two_pass)
       for (int i = tid; i < F1; i += BLOCK) s_coarse[i] = 0;
     lds_barrier();
-    for (uint32_t r = 0; r < nrounds; ++r) {
-      uint32_t low[4];
-      bool valid[4];
-      load4_low<KB, VEC>(T.keys, lo + (r * BLOCK + tid) * 4, hi, low, valid);
-#pragma unroll
-      for (int c = 0; c < 4; ++c)
-        if (valid[c]) atomicAdd(&s_bins[(low[c] >> pp.shift0) & pmask], 1u);
-    }
+    for (uint32_t f = 0; f < nfull; ++f) count_batch(f);
+    for (uint32_t r = nfull * BATCH; r < nrounds; ++r) count_round(r);
     lds_barrier();
     uint32_t* row = T.mat + size_t(seg) * P;
     for (int i = tid; i < P; i += BLOCK) {
@@ -161,20 +198,8 @@ __global__ __launch_bounds__(kHistBlock) void k_hist(PartTable t0, PartTable t1,
     if (two_pass)
       for (int i = tid; i < F1; i += BLOCK) s_coarse[i] = 0;
     lds_barrier();
-    constexpr uint32_t kEpochRounds = 15;  // 15 * 4096 = 61440 keys < 65536
-    for (uint32_t r0 = 0; r0 < nrounds; r0 += kEpochRounds) {
-      const uint32_t r1 = min(nrounds, r0 + kEpochRounds);
-      for (uint32_t r = r0; r < r1; ++r) {
-        uint32_t low[4];
-        bool valid[4];
-        load4_low<KB, VEC>(T.keys, lo + (r * BLOCK + tid) * 4, hi, low, valid);
-#pragma unroll
-        for (int c = 0; c < 4; ++c)
-          if (valid[c]) {
-            const uint32_t d = (low[c] >> pp.shift0) & pmask;
-            atomicAdd(&s_bins[d >> 1], 1u << ((d & 1u) << 4));
-          }
-      }
+    // Folds the 16-bit fields into the per-thread accumulators and clears them.
+    auto fold = [&]() {
       lds_barrier();
 #pragma unroll
       for (int k = 0; k < WPT; ++k) {
@@ -184,6 +209,17 @@ __global__ __launch_bounds__(kHistBlock) void k_hist(PartTable t0, PartTable t1,
         s_bins[tid + BLOCK * k] = 0;
       }
       lds_barrier();
+    };
+    for (uint32_t f0 = 0; f0 < nfull; f0 += kEpochBatches) {
+      const uint32_t f1 = min(nfull, f0 + kEpochBatches);
+      for (uint32_t f = f0; f < f1; ++f) count_batch(f);
+      fold();
+    }
+    constexpr uint32_t kEpochRounds = kEpochBatches * BATCH;
+    for (uint32_t r0 = nfull * BATCH; r0 < nrounds; r0 += kEpochRounds) {
+      const uint32_t r1 = min(nrounds, r0 + kEpochRounds);
+      for (uint32_t r = r0; r < r1; ++r) count_round(r);
+      fold();
     }
     uint2* row = reinterpret_cast<uint2*>(T.mat + size_t(seg) * P);
 #pragma unroll
@@ -387,22 +423,37 @@ hipError_t launch_join_plan(const Layout& L, int base, int64_t slice, hipStream_
 }
 
 // ---------------------------------------------------------------------------
-// Shared tile scatter: E items per thread already in registers (digit 0xFFFFFFFF
-// marks an empty item).  Ranks by LDS atomics, scans the tile histogram,
-// advances the per-digit global cursors s_cur, stages the tile in LDS in digit
-// order and writes it out in coalesced runs.  Ends with an LDS barrier.
+// Shared tile scatter: E tuples per thread already in registers (digit
+// 0xFFFFFFFF marks an empty item) are ranked with LDS atomics (rank_tile), the
+// tile histogram is scanned and the tile staged in LDS in digit order
+// (scan_stage_tile), and the staged tile is written out in coalesced runs
+// (flush_full / flush_tile).  The next tile's loads are issued between the
+// ranking and the scan, when the fewest registers are live.  Tile t is
+// flushed at the start of tile t + 1, before the loads of tile t + 2: the
+// memory counter completes in order, so the wait for those loads finds the
+// stores long complete instead of waiting for them at once.  The barrier that
+// follows the clearing of s_cnt for tile t + 1 orders the flush's LDS reads
+// before tile t + 1 rewrites s_delta and the stage.
 // ---------------------------------------------------------------------------
-template <int KB, int BLOCK, int E>
-__device__ __forceinline__ void scatter_tile(const typename KeyT<KB>::key_t (&key)[E], const int32_t (&pay)[E],
-                                             const uint32_t (&dg)[E], uint32_t len, int F, int shift,
-                                             uint32_t mask, typename KeyT<KB>::tup_t* s_stage, uint32_t* s_cnt,
-                                             uint32_t* s_base, uint32_t* s_cur, uint32_t* s_delta, uint32_t* s_tmp,
-                                             typename KeyT<KB>::tup_t* out, uint32_t n_out) {
-  using KTr = KeyT<KB>;
-  const int tid = threadIdx.x;
-  uint32_t rk[E];
+// Ranks of a tile's items within their digits, from LDS atomics on the
+// cleared s_cnt: pk = digit << 16 | rank (digits < 4096, ranks < 65536), or
+// 0xFFFFFFFF for an empty item.
+template <int E>
+__device__ __forceinline__ void rank_tile(const uint32_t (&dg)[E], uint32_t* s_cnt, uint32_t (&pk)[E]) {
 #pragma unroll
-  for (int e = 0; e < E; ++e) rk[e] = dg[e] != 0xFFFFFFFFu ? atomicAdd(&s_cnt[dg[e]], 1u) : 0u;
+  for (int e = 0; e < E; ++e)
+    pk[e] = dg[e] != 0xFFFFFFFFu ? (dg[e] << 16) | atomicAdd(&s_cnt[dg[e]], 1u) : 0xFFFFFFFFu;
+}
+
+// After rank_tile: scans the tile histogram, advances the per-digit global
+// cursors s_cur, records s_delta (global minus staged position per digit) and
+// stages the tile in digit order.  Three barriers; ends with one.
+template <int KB, int BLOCK, int E>
+__device__ __forceinline__ void scan_stage_tile(const typename KeyT<KB>::tup_t (&tup)[E], const uint32_t (&pk)[E],
+                                                int F, typename KeyT<KB>::tup_t* s_stage, const uint32_t* s_cnt,
+                                                uint32_t* s_base, uint32_t* s_cur, uint32_t* s_delta,
+                                                uint32_t* s_tmp) {
+  const int tid = threadIdx.x;
   lds_barrier();
   // Exclusive scan of the F digit counts: each thread owns DPT consecutive digits.
   const int DPT = F > BLOCK ? F / BLOCK : 1;
@@ -425,15 +476,40 @@ __device__ __forceinline__ void scatter_tile(const typename KeyT<KB>::key_t (&ke
   lds_barrier();
 #pragma unroll
   for (int e = 0; e < E; ++e)
-    if (dg[e] != 0xFFFFFFFFu) s_stage[s_base[dg[e]] + rk[e]] = KTr::make(key[e], pay[e]);
+    if (pk[e] != 0xFFFFFFFFu) s_stage[s_base[pk[e] >> 16] + (pk[e] & 0xFFFFu)] = tup[e];
   lds_barrier();
+}
+
+// Flush of a full tile: every thread stores exactly E tuples, unconditionally,
+// so the compiler knows how many memory operations follow the next tile's
+// loads and waits for those alone.  A destination past the table (impossible
+// with exact offsets) is clamped to its last row rather than faulting.
+template <int KB, int BLOCK, int E>
+__device__ __forceinline__ void flush_full(int shift, uint32_t mask, const typename KeyT<KB>::tup_t* s_stage,
+                                           const uint32_t* s_delta, typename KeyT<KB>::tup_t* out, uint32_t n_out) {
+  using KTr = KeyT<KB>;
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const uint32_t j = uint32_t(e) * BLOCK + tid;
+    const typename KTr::tup_t t = s_stage[j];
+    const uint32_t d = s_delta[digit_of(KTr::low32(KTr::key(t)), shift, mask)] + j;
+    out[d < n_out ? d : n_out - 1] = t;
+  }
+}
+
+// Flush of a partial tile of len tuples (the last of a range).
+template <int KB, int BLOCK, int E>
+__device__ __forceinline__ void flush_tile(uint32_t len, int shift, uint32_t mask,
+                                           const typename KeyT<KB>::tup_t* s_stage, const uint32_t* s_delta,
+                                           typename KeyT<KB>::tup_t* out, uint32_t n_out) {
+  using KTr = KeyT<KB>;
+  const int tid = threadIdx.x;
   for (uint32_t j = tid; j < len; j += BLOCK) {
     const typename KTr::tup_t t = s_stage[j];
-    const uint32_t d = digit_of(KTr::low32(KTr::key(t)), shift, mask);
-    const uint32_t pos = s_delta[d] + j;
-    if (pos < n_out) out[pos] = t;  // always true for exact offsets; guards the table bounds
+    const uint32_t d = s_delta[digit_of(KTr::low32(KTr::key(t)), shift, mask)] + j;
+    if (d < n_out) out[d] = t;  // always in range for exact offsets; guards the table
   }
-  lds_barrier();
 }
 
 // LDS layout of the scatter kernels: stage[TILE] then cnt, base, cur, delta [F] and tmp.
@@ -443,24 +519,24 @@ __host__ __device__ constexpr size_t scatter_lds_bytes(int F) {
 }
 
 // ---------------------------------------------------------------------------
-// k_pass1: one workgroup per segment; digit (key >> b2) & (F1 - 1).
+// k_pass1: one workgroup per segment; digit (key >> b2) & (F1 - 1).  Full
+// tiles load without any condition, so the next tile's loads stay in flight
+// until their registers are used; only the segment's partial last tile loads
+// under bounds checks.  PAY0 / PAY1: table 0 / 1 has a payload column (else
+// the payload is the row id).
 // ---------------------------------------------------------------------------
-template <int KB, bool VEC, int BLOCK, int TILE>
-__global__ __launch_bounds__(BLOCK) void k_pass1(PartTable t0, PartTable t1, PartParams pp) {
+template <int KB, bool VEC, bool PAY, int BLOCK, int TILE>
+__device__ __forceinline__ void pass1_body(const PartTable& T, const PartParams& pp, uint32_t b) {
   using KTr = KeyT<KB>;
   using key_t = typename KTr::key_t;
   using tup_t = typename KTr::tup_t;
   constexpr int E = TILE / BLOCK;
   static_assert(E % 4 == 0, "tile must give each thread a multiple of 4 items");
-  const PartTable T = blockIdx.y ? t1 : t0;
-  const uint32_t b = blockIdx.x;
-  if (b >= T.nseg) return;
   const uint32_t S = T.nseg / kNumXcd;
   const uint32_t seg = (b % kNumXcd) * S + b / kNumXcd;
   const uint32_t lo = seg_bound(T.n, T.nseg, seg), hi = seg_bound(T.n, T.nseg, seg + 1);
   const int F = pp.F1, shift = pp.shift0 + pp.b2;
   const uint32_t mask = uint32_t(F - 1);
-  const bool haspay = T.pay != nullptr;
 
   extern __shared__ __align__(16) uint8_t s_raw[];
   tup_t* s_stage = reinterpret_cast<tup_t*>(s_raw);
@@ -476,12 +552,12 @@ __global__ __launch_bounds__(BLOCK) void k_pass1(PartTable t0, PartTable t1, Par
   tup_t* out = static_cast<tup_t*>(T.buf1);
   const key_t* keys = static_cast<const key_t*>(T.keys);
 
-  // Loads the 4-item groups of one tile into registers (keys and payloads).
-  auto load_tile = [&](uint32_t tile_lo, key_t (&key)[E], int32_t (&pay)[E]) {
+  // Item e4 * 4 + c of a thread is row tile_lo + (e4 * BLOCK + tid) * 4 + c.
+  auto load_full = [&](uint32_t tile_lo, key_t (&key)[E], int32_t (&pay)[E]) {
 #pragma unroll
     for (int e4 = 0; e4 < E / 4; ++e4) {
       const uint32_t i = tile_lo + (uint32_t(e4) * BLOCK + tid) * 4;
-      if (VEC && i + 4 <= hi) {
+      if (VEC) {
         if (KB == 4) {
           const u32x4 v = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(keys + i));
           key[e4 * 4 + 0] = key_t(v.x); key[e4 * 4 + 1] = key_t(v.y);
@@ -494,37 +570,33 @@ __global__ __launch_bounds__(BLOCK) void k_pass1(PartTable t0, PartTable t1, Par
           key[e4 * 4 + 2] = key_t(uint64_t(c.x) | (uint64_t(c.y) << 32));
           key[e4 * 4 + 3] = key_t(uint64_t(c.z) | (uint64_t(c.w) << 32));
         }
-        if (haspay) {
+        if (PAY) {
           const u32x4 v = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(T.pay + i));
           pay[e4 * 4 + 0] = int32_t(v.x); pay[e4 * 4 + 1] = int32_t(v.y);
           pay[e4 * 4 + 2] = int32_t(v.z); pay[e4 * 4 + 3] = int32_t(v.w);
-        } else {
-#pragma unroll
-          for (int c = 0; c < 4; ++c) pay[e4 * 4 + c] = int32_t(i + c);
         }
       } else {
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
-          const bool valid = i + c < hi;
-          key[e4 * 4 + c] = valid ? __builtin_nontemporal_load(keys + i + c) : key_t(0);
-          pay[e4 * 4 + c] = haspay ? (valid ? __builtin_nontemporal_load(T.pay + i + c) : 0) : int32_t(i + c);
+          key[e4 * 4 + c] = __builtin_nontemporal_load(keys + i + c);
+          if (PAY) pay[e4 * 4 + c] = __builtin_nontemporal_load(T.pay + i + c);
         }
       }
     }
+  };  // without a payload column pay is left alone: the row id is formed when staging
+  auto load_tail = [&](uint32_t tile_lo, key_t (&key)[E], int32_t (&pay)[E]) {
+#pragma unroll
+    for (int e4 = 0; e4 < E / 4; ++e4) {
+      const uint32_t i = tile_lo + (uint32_t(e4) * BLOCK + tid) * 4;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const bool valid = i + c < hi;
+        key[e4 * 4 + c] = valid ? keys[i + c] : key_t(0);
+        pay[e4 * 4 + c] = PAY ? (valid ? T.pay[i + c] : 0) : int32_t(i + c);
+      }
+    }
   };
-  key_t key[E];
-  int32_t pay[E];
-  if (lo < hi) load_tile(lo, key, pay);
-  for (uint32_t tile_lo = lo; tile_lo < hi; tile_lo += TILE) {
-    const uint32_t len = min(uint32_t(TILE), hi - tile_lo);
-    // next tile in flight while this one is ranked, staged and written
-    key_t nkey[E];
-    int32_t npay[E];
-    const uint32_t next_lo = tile_lo + TILE;
-    if (next_lo < hi) load_tile(next_lo, nkey, npay);
-    for (int d = tid; d < F; d += BLOCK) s_cnt[d] = 0;
-    lds_barrier();
-    uint32_t dg[E];
+  auto digits = [&](uint32_t tile_lo, const key_t (&key)[E], uint32_t (&dg)[E]) {
 #pragma unroll
     for (int e4 = 0; e4 < E / 4; ++e4) {
       const uint32_t i = tile_lo + (uint32_t(e4) * BLOCK + tid) * 4;
@@ -532,24 +604,97 @@ __global__ __launch_bounds__(BLOCK) void k_pass1(PartTable t0, PartTable t1, Par
       for (int c = 0; c < 4; ++c)
         dg[e4 * 4 + c] = i + c < hi ? digit_of(KTr::low32(key[e4 * 4 + c]), shift, mask) : 0xFFFFFFFFu;
     }
-    scatter_tile<KB, BLOCK, E>(key, pay, dg, len, F, shift, mask, s_stage, s_cnt, s_base, s_cur, s_delta, s_tmp,
-                               out, T.n);
+  };
+
+  // Full tiles.  Tile 0 is peeled so that every later iteration flushes the
+  // previous tile (E stores per thread) and loads the next one (clamped to the
+  // last full tile, which the final iteration reloads) without conditions:
+  // with fixed counts the compiler's waits for a tile's loads leave the
+  // younger stores and loads in flight.
+  const uint32_t nfull = (hi - lo) / uint32_t(TILE);
+  // One full tile held in (ck, cp): flush the previous tile (flush_prev),
+  // rank this one, issue the next tile's loads into (nk, np), scan and stage.
+  // The caller alternates two register sets, so a loaded tile is never copied
+  // (a copy would wait for the loads).
+  auto full_tile = [&](uint32_t t, auto flush_prev, const key_t (&ck)[E], const int32_t (&cp)[E], key_t (&nk)[E],
+                       int32_t (&np)[E]) {
+    if (decltype(flush_prev)::value) flush_full<KB, BLOCK, E>(shift, mask, s_stage, s_delta, out, T.n);
+    for (int d = tid; d < F; d += BLOCK) s_cnt[d] = 0;
+    lds_barrier();
+    uint32_t pk[E];
+    {
+      uint32_t dg[E];
 #pragma unroll
-    for (int e = 0; e < E; ++e) {
-      key[e] = nkey[e];
-      pay[e] = npay[e];
+      for (int e = 0; e < E; ++e) dg[e] = digit_of(KTr::low32(ck[e]), shift, mask);
+      rank_tile<E>(dg, s_cnt, pk);
     }
+    const uint32_t tn = t + 1 < nfull ? t + 1 : t;
+    load_full(lo + tn * uint32_t(TILE), nk, np);  // next tile in flight during the scan and staging
+    tup_t tup[E];
+    const uint32_t tile_lo = lo + t * uint32_t(TILE);
+#pragma unroll
+    for (int e = 0; e < E; ++e)
+      tup[e] = KTr::make(ck[e], PAY ? cp[e] : int32_t(tile_lo + (uint32_t(e / 4) * BLOCK + tid) * 4 + (e % 4)));
+    scan_stage_tile<KB, BLOCK, E>(tup, pk, F, s_stage, s_cnt, s_base, s_cur, s_delta, s_tmp);
+  };
+  key_t key[E];
+  int32_t pay[E];
+  if (nfull > 0) {
+    key_t key2[E];
+    int32_t pay2[E];
+    load_full(lo, key, pay);
+    full_tile(0u, std::false_type{}, key, pay, key2, pay2);
+    uint32_t t = 1;
+    for (; t + 1 < nfull; t += 2) {
+      full_tile(t, std::true_type{}, key2, pay2, key, pay);
+      full_tile(t + 1, std::true_type{}, key, pay, key2, pay2);
+    }
+    if (t < nfull) full_tile(t, std::true_type{}, key2, pay2, key, pay);
+    flush_full<KB, BLOCK, E>(shift, mask, s_stage, s_delta, out, T.n);
   }
+  // The partial last tile.
+  const uint32_t tail_lo = lo + nfull * uint32_t(TILE);
+  if (tail_lo < hi) {
+    load_tail(tail_lo, key, pay);
+    for (int d = tid; d < F; d += BLOCK) s_cnt[d] = 0;
+    lds_barrier();
+    uint32_t dg[E], pk[E];
+    digits(tail_lo, key, dg);
+    rank_tile<E>(dg, s_cnt, pk);
+    tup_t tup[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) tup[e] = KTr::make(key[e], pay[e]);
+    scan_stage_tile<KB, BLOCK, E>(tup, pk, F, s_stage, s_cnt, s_base, s_cur, s_delta, s_tmp);
+    flush_tile<KB, BLOCK, E>(hi - tail_lo, shift, mask, s_stage, s_delta, out, T.n);
+  }
+}
+
+template <int KB, bool VEC, bool PAY0, bool PAY1, int BLOCK, int TILE>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void k_pass1(PartTable t0, PartTable t1, PartParams pp) {
+  const uint32_t b = blockIdx.x;
+  if (blockIdx.y == 0) {
+    if (b < t0.nseg) pass1_body<KB, VEC, PAY0, BLOCK, TILE>(t0, pp, b);
+  } else {
+    if (b < t1.nseg) pass1_body<KB, VEC, PAY1, BLOCK, TILE>(t1, pp, b);
+  }
+}
+
+template <int KB, bool VEC, bool PAY0, bool PAY1>
+static hipError_t pass1_launch(const Layout& L, hipStream_t s) {
+  constexpr int TILE = KB == 4 ? kScatterTile4 : kScatterTile8;
+  const uint32_t nseg = L.t[0].nseg > L.t[1].nseg ? L.t[0].nseg : L.t[1].nseg;
+  const size_t lds = scatter_lds_bytes<KB, TILE>(L.pp.F1);
+  hipLaunchKernelGGL((k_pass1<KB, VEC, PAY0, PAY1, kScatterBlock, TILE>), dim3(nseg, L.ntables),
+                     dim3(kScatterBlock), lds, s, L.t[0], L.t[1], L.pp);
+  return hipGetLastError();
 }
 
 template <int KB, bool VEC>
 static hipError_t pass1_dispatch(const Layout& L, hipStream_t s) {
-  constexpr int TILE = KB == 4 ? kScatterTile4 : kScatterTile8;
-  const uint32_t nseg = L.t[0].nseg > L.t[1].nseg ? L.t[0].nseg : L.t[1].nseg;
-  const size_t lds = scatter_lds_bytes<KB, TILE>(L.pp.F1);
-  hipLaunchKernelGGL((k_pass1<KB, VEC, kScatterBlock, TILE>), dim3(nseg, L.ntables), dim3(kScatterBlock), lds, s,
-                     L.t[0], L.t[1], L.pp);
-  return hipGetLastError();
+  const bool p0 = L.t[0].pay != nullptr;
+  const bool p1 = L.ntables > 1 && L.t[1].pay != nullptr;
+  if (p0) return p1 ? pass1_launch<KB, VEC, true, true>(L, s) : pass1_launch<KB, VEC, true, false>(L, s);
+  return p1 ? pass1_launch<KB, VEC, false, true>(L, s) : pass1_launch<KB, VEC, false, false>(L, s);
 }
 
 static bool aligned16(const void* p) { return p == nullptr || (reinterpret_cast<uintptr_t>(p) % 16) == 0; }
@@ -565,7 +710,7 @@ hipError_t launch_pass1(const Layout& L, int key_bytes, hipStream_t s) {
 // k_pass2: items (p1, segment group) -> split by digit key & (F2 - 1).
 // ---------------------------------------------------------------------------
 template <int KB, int BLOCK, int TILE>
-__global__ __launch_bounds__(BLOCK) void k_pass2(PartTable t0, PartTable t1, PartParams pp) {
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void k_pass2(PartTable t0, PartTable t1, PartParams pp) {
   using KTr = KeyT<KB>;
   using key_t = typename KTr::key_t;
   using tup_t = typename KTr::tup_t;
@@ -590,6 +735,13 @@ __global__ __launch_bounds__(BLOCK) void k_pass2(PartTable t0, PartTable t1, Par
   const tup_t* in = static_cast<const tup_t*>(T.buf1);
   tup_t* out = static_cast<tup_t*>(T.buf2);
 
+  auto digits = [&](const tup_t (&tv)[E], uint32_t tile_lo, uint32_t hi, uint32_t (&dg)[E]) {
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const bool valid = tile_lo + uint32_t(e) * BLOCK + tid < hi;
+      dg[e] = valid ? digit_of(KTr::low32(KTr::key(tv[e])), pp.shift0, mask) : 0xFFFFFFFFu;
+    }
+  };
   for (uint32_t k = j; k < items; k += G) {
     const int p1 = x + kNumXcd * int(k / NG);
     const uint32_t g = k % NG;
@@ -599,39 +751,56 @@ __global__ __launch_bounds__(BLOCK) void k_pass2(PartTable t0, PartTable t1, Par
     const uint32_t in_lo = base1 + T.matc[size_t(seg_lo) * F1 + p1];
     const uint32_t in_hi = seg_hi < nseg ? base1 + T.matc[size_t(seg_hi) * F1 + p1] : T.off[q0 + F2];
     if (in_lo >= in_hi || in_hi > T.n) continue;
+    // s_cur was last used by the previous tile's scan, before its staging barrier
     for (int d = tid; d < F2; d += BLOCK) s_cur[d] = T.off[q0 + d] + T.mat[size_t(seg_lo) * P + q0 + d];
-    tup_t tv[E];
+    // Full tiles, pipelined as in pass 1: tile 0 peeled, the next tile's
+    // loads clamped to the last full tile, E stores per thread per flush.
+    const uint32_t nfull = (in_hi - in_lo) / uint32_t(TILE);
+    auto full_tile = [&](uint32_t t, auto flush_prev, const tup_t (&cur)[E], tup_t (&nxt)[E]) {
+      if (decltype(flush_prev)::value) flush_full<KB, BLOCK, E>(pp.shift0, mask, s_stage, s_delta, out, T.n);
+      for (int d = tid; d < F2; d += BLOCK) s_cnt[d] = 0;
+      lds_barrier();
+      uint32_t pk[E];
+      {
+        uint32_t dg[E];
 #pragma unroll
-    for (int e = 0; e < E; ++e) {
-      const uint32_t i = in_lo + uint32_t(e) * BLOCK + tid;
-      if (i < in_hi) tv[e] = in[i];
+        for (int e = 0; e < E; ++e) dg[e] = digit_of(KTr::low32(KTr::key(cur[e])), pp.shift0, mask);
+        rank_tile<E>(dg, s_cnt, pk);
+      }
+      const uint32_t next_lo = in_lo + (t + 1 < nfull ? t + 1 : t) * uint32_t(TILE);
+#pragma unroll
+      for (int e = 0; e < E; ++e) nxt[e] = in[next_lo + uint32_t(e) * BLOCK + tid];  // next tile in flight
+      scan_stage_tile<KB, BLOCK, E>(cur, pk, F2, s_stage, s_cnt, s_base, s_cur, s_delta, s_tmp);
+    };
+    tup_t tv[E];
+    if (nfull > 0) {
+      tup_t tw[E];
+#pragma unroll
+      for (int e = 0; e < E; ++e) tv[e] = in[in_lo + uint32_t(e) * BLOCK + tid];
+      full_tile(0u, std::false_type{}, tv, tw);
+      uint32_t t = 1;
+      for (; t + 1 < nfull; t += 2) {
+        full_tile(t, std::true_type{}, tw, tv);
+        full_tile(t + 1, std::true_type{}, tv, tw);
+      }
+      if (t < nfull) full_tile(t, std::true_type{}, tw, tv);
+      flush_full<KB, BLOCK, E>(pp.shift0, mask, s_stage, s_delta, out, T.n);
     }
-    for (uint32_t tile_lo = in_lo; tile_lo < in_hi; tile_lo += TILE) {
-      const uint32_t len = min(uint32_t(TILE), in_hi - tile_lo);
-      tup_t tn[E];
-      const uint32_t next_lo = tile_lo + TILE;
+    // The partial last tile of the item.
+    const uint32_t tail_lo = in_lo + nfull * uint32_t(TILE);
+    if (tail_lo < in_hi) {
 #pragma unroll
       for (int e = 0; e < E; ++e) {
-        const uint32_t i = next_lo + uint32_t(e) * BLOCK + tid;
-        if (next_lo < in_hi && i < in_hi) tn[e] = in[i];
+        const uint32_t i = tail_lo + uint32_t(e) * BLOCK + tid;
+        if (i < in_hi) tv[e] = in[i];  // lanes past the end are marked empty by their digit
       }
       for (int d = tid; d < F2; d += BLOCK) s_cnt[d] = 0;
       lds_barrier();
-      key_t key[E];
-      int32_t pay[E];
-      uint32_t dg[E];
-#pragma unroll
-      for (int e = 0; e < E; ++e) {
-        const uint32_t i = tile_lo + uint32_t(e) * BLOCK + tid;
-        const bool valid = i < in_hi;
-        key[e] = valid ? KTr::key(tv[e]) : key_t(0);
-        pay[e] = valid ? KTr::pay(tv[e]) : 0;
-        dg[e] = valid ? digit_of(KTr::low32(key[e]), pp.shift0, mask) : 0xFFFFFFFFu;
-      }
-      scatter_tile<KB, BLOCK, E>(key, pay, dg, len, F2, pp.shift0, mask, s_stage, s_cnt, s_base, s_cur, s_delta,
-                                 s_tmp, out, T.n);
-#pragma unroll
-      for (int e = 0; e < E; ++e) tv[e] = tn[e];
+      uint32_t dg[E], pk[E];
+      digits(tv, tail_lo, in_hi, dg);
+      rank_tile<E>(dg, s_cnt, pk);
+      scan_stage_tile<KB, BLOCK, E>(tv, pk, F2, s_stage, s_cnt, s_base, s_cur, s_delta, s_tmp);
+      flush_tile<KB, BLOCK, E>(in_hi - tail_lo, pp.shift0, mask, s_stage, s_delta, out, T.n);
     }
   }
 }

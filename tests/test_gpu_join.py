@@ -189,3 +189,26 @@ def test_config2_full_size_properties(pj, torch):
     seen = torch.zeros(pk.numel(), dtype=torch.int32, device="cuda")
     seen.index_add_(0, p, torch.ones_like(pairs[:, 1]))
     assert bool((seen == match.int()).all().item())
+
+
+def test_gather_columns(pj, oracle, torch):
+    """Gather map: payload columns of the joined rows, int32/int64/float64."""
+    rng = np.random.default_rng(24)
+    bk = rng.permutation(1 << 16).astype(np.int32)
+    pk = rng.integers(0, 1 << 17, 300000).astype(np.int32)
+    d = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    pairs = pj.partitioned_join(d(bk), d(pk))
+    pr = pairs.cpu().numpy().astype(np.int64)
+    bcol = rng.integers(-2**40, 2**40, len(bk)).astype(np.int64)
+    pcol = rng.integers(-2**31, 2**31 - 1, len(pk)).astype(np.int32)
+    gb, gp = pj.gather(pairs, d(bcol), d(pcol))
+    assert np.array_equal(gb.cpu().numpy(), bcol[pr[:, 0]])
+    assert np.array_equal(gp.cpu().numpy(), pcol[pr[:, 1]])
+    fcol = rng.standard_normal(len(pk))
+    none_b, gf = pj.gather(pairs, None, d(fcol))
+    assert none_b is None and np.array_equal(gf.cpu().numpy(), fcol[pr[:, 1]])
+    # every gathered build key equals the probe key it joined
+    kb, kp = pj.gather(pairs, d(bk), d(pk))
+    assert bool((kb == kp).all().item())
+    e0, e1 = pj.gather(pairs[:0], d(bcol), d(pcol))
+    assert e0.numel() == 0 and e1.numel() == 0
