@@ -363,9 +363,9 @@ pj_status pj_gather(const pj_pair* pairs, int64_t n, const void* build_col, int3
     return fail(PJ_ERR_INVALID_ARG, "build column width must be 4 or 8 (got %d)", build_bytes);
   if (probe_col && probe_bytes != 4 && probe_bytes != 8)
     return fail(PJ_ERR_INVALID_ARG, "probe column width must be 4 or 8 (got %d)", probe_bytes);
+  if (n == 0 || (!build_col && !probe_col)) return PJ_OK;
   if ((build_col && !out_build) || (probe_col && !out_probe))
     return fail(PJ_ERR_INVALID_ARG, "a gathered column needs an output buffer");
-  if (n == 0 || (!build_col && !probe_col)) return PJ_OK;
   PJ_HIP(gather_columns(pairs, n, build_col, build_col ? build_bytes : 4, probe_col, probe_col ? probe_bytes : 4,
                         out_build, out_probe, stream));
   return PJ_OK;
