@@ -69,7 +69,8 @@ struct JoinArgs {
   pj_pair* spill;          // kJoinGrid * kOutChunk pairs
   uint32_t* chunk_ctr;     // claimed output chunks
   uint64_t* wg_last;       // [grid] (chunk << 32 | fill) or ~0 when no chunk; fill <= kOutChunk
-  int table_cap;           // LDS slots available for the build table
+  int table_cap;           // LDS slots available for the build table (a multiple of 4)
+  int bucket_slots;        // keys per 16-byte bucket: 4 (4-byte keys) or 2 (8-byte keys)
   uint32_t nr, ns;         // table sizes (bounds for the item ranges)
   const uint4* desc;       // expanded work item descriptors (2 uint4 per item)
   uint64_t desc_cap;       // items with a descriptor; later items are decoded by search
@@ -130,11 +131,12 @@ size_t join_generic_ws_bytes(int B);
 // the rest of kJoinLdsBytes); the build chunk is at most half of it.
 int join_table_capacity(int key_bytes);
 
-// Slots of the table of a build chunk of c tuples: 8c + 2 (occupancy <= 1/8),
-// at most cap.  c <= cap/2 - 1 keeps at least 2c + 2, so every probe walk
-// ends at an empty slot.
+// Slots of the table of a build chunk of c tuples: 8c + 2 rounded up to whole
+// 16-byte key buckets (occupancy <= 1/8), at most cap (a multiple of 4).
+// c <= cap/2 - 1 keeps at least 2c + 2, so every probe walk ends at an empty
+// slot.
 __host__ __device__ inline uint32_t join_table_slots(uint32_t c, uint32_t cap) {
-  const uint64_t t = 8ull * c + 2ull;
+  const uint64_t t = (8ull * c + 2ull + 3ull) & ~3ull;
   return t < cap ? uint32_t(t) : cap;
 }
 

@@ -7,14 +7,16 @@
 //                handle_skewed_data reads the block count back to the host,
 //                src/skewed_data_handler.cuh:83, and splits only the build side).
 //   k_join       persistent workgroups; per item: LDS open-addressing table of
-//                8*|chunk| + 2 slots (occupancy <= 1/8), MurmurHash3 seed 0,
-//                slot = |(int)h % size| with int_fastdiv, linear probing with
-//                LDS CAS (src/hash_join_kernels.cuh:84-129), then the probe
-//                slice walks the table (:134-181).  Each slot holds the key and
-//                the build payload together, so a match needs no second read.
-//                Probes of one thread walk interleaved: a step advances every
-//                unfinished probe by one slot, so a wave waits for its longest
-//                chain once per round instead of once per probe.  Matches are
+//                8*|chunk| + 2 slots (occupancy <= 1/8) in 16-byte key buckets,
+//                MurmurHash3 seed 0, bucket = |(int)h % buckets| with
+//                int_fastdiv, linear probing with LDS CAS from the bucket's
+//                first slot (src/hash_join_kernels.cuh:84-129), then the probe
+//                slice walks the table (:134-181).  Slots fill each bucket in
+//                order, so a probe reads one whole bucket per step (one 16-byte
+//                LDS read) and stops at a match or at an empty slot: nearly
+//                every probe ends in its first bucket.  Probes of one thread
+//                walk interleaved, so a wave waits for its longest chain once
+//                per round instead of once per probe.  Matches are
 //                staged in LDS and flushed into output chunks of kOutChunk
 //                pairs that each workgroup claims with one global atomic (a
 //                same-address device atomic costs ~12 ns on MI355X, so claims
@@ -31,12 +33,18 @@ namespace pj {
 
 static int join_probe_per_thread(int key_bytes) { return key_bytes == 4 ? kJoinProbe4 : kJoinProbe8; }
 static int join_stage_pairs(int key_bytes) { return join_probe_per_thread(key_bytes) * kJoinBlock; }
-static int join_slot_bytes(int key_bytes) { return key_bytes == 4 ? 8 : 16; }
+static int join_slot_bytes(int key_bytes) { return key_bytes + 4; }  // key + build payload
 
-int join_table_capacity(int key_bytes) {
-  const int bytes = kJoinLdsBytes - join_stage_pairs(key_bytes) * 8;
-  return (bytes / join_slot_bytes(key_bytes)) & ~1;
+// Slots that fit next to the stage, a multiple of 4; for 8-byte keys at most
+// 4096, so a build chunk (at most half the table) fits NB = 4 tuples per thread.
+static constexpr int table_cap_of(int key_bytes) {
+  const int c = ((kJoinLdsBytes - (key_bytes == 4 ? kJoinProbe4 : kJoinProbe8) * kJoinBlock * 8) / (key_bytes + 4)) & ~3;
+  return key_bytes == 8 && c > 4096 ? 4096 : c;
 }
+static_assert(6 * kJoinBlock >= table_cap_of(4) / 2, "NB = 6 covers the largest 4-byte-key chunk");
+static_assert(4 * kJoinBlock >= table_cap_of(8) / 2, "NB = 4 covers the largest 8-byte-key chunk");
+
+int join_table_capacity(int key_bytes) { return table_cap_of(key_bytes); }
 
 static size_t join_lds_bytes(int key_bytes, int table_slots) {
   return size_t(table_slots) * join_slot_bytes(key_bytes) + size_t(join_stage_pairs(key_bytes)) * 8;
@@ -123,16 +131,24 @@ __device__ __forceinline__ uint64_t lds_cas(uint64_t* p, uint64_t cmp, uint64_t 
                             (unsigned long long)val));
 }
 
-// Key word(s) and payload word of an LDS table slot (the slot has the tuple layout).
-__device__ __forceinline__ uint32_t* slot_key(uint2* s) { return &s->x; }
-__device__ __forceinline__ uint64_t* slot_key(uint4* s) { return reinterpret_cast<uint64_t*>(s); }
-__device__ __forceinline__ void slot_set_pay(uint2* s, int32_t p) { s->y = uint32_t(p); }
-__device__ __forceinline__ void slot_set_pay(uint4* s, int32_t p) { s->z = uint32_t(p); }
+// The keys of one 16-byte bucket.
+__device__ __forceinline__ void bucket_keys(const uint4 v, uint32_t (&k)[4]) {
+  k[0] = v.x; k[1] = v.y; k[2] = v.z; k[3] = v.w;
+}
+__device__ __forceinline__ void bucket_keys(const uint4 v, uint64_t (&k)[2]) {
+  k[0] = uint64_t(v.x) | (uint64_t(v.y) << 32);
+  k[1] = uint64_t(v.z) | (uint64_t(v.w) << 32);
+}
+__device__ __forceinline__ uint4 bucket_fill(uint32_t k) { return make_uint4(k, k, k, k); }
+__device__ __forceinline__ uint4 bucket_fill(uint64_t k) {
+  return make_uint4(uint32_t(k), uint32_t(k >> 32), uint32_t(k), uint32_t(k >> 32));
+}
 
 __device__ __forceinline__ uint32_t uniform(uint32_t x) { return __builtin_amdgcn_readfirstlane(x); }
 
 // One work item: partition q, build chunk [rlo, rhi), probe slice [slo, shi);
-// the LDS table of the chunk has fd.d = join_table_slots(rhi - rlo) slots.
+// the LDS table of the chunk has fd.d = join_table_slots(rhi - rlo) /
+// bucket_slots buckets.
 // Stored as two uint4 words: {q, rlo, rhi, slo}, {shi, magic, shift, add_sign}.
 struct ItemDesc {
   uint32_t q, rlo, rhi, slo, shi, valid;
@@ -160,7 +176,7 @@ __device__ __forceinline__ ItemDesc decode_item(const JoinArgs& A, uint64_t item
   const uint64_t slo64 = uint64_t(s0) + sl * uint64_t(A.slice);
   d.slo = uint32_t(min(uint64_t(s1), slo64));
   d.shi = uint32_t(min(uint64_t(s1), slo64 + uint64_t(A.slice)));
-  d.fd = fastdiv_make(int32_t(join_table_slots(d.rhi - d.rlo, uint32_t(A.table_cap))));
+  d.fd = fastdiv_make(int32_t(join_table_slots(d.rhi - d.rlo, uint32_t(A.table_cap)) / uint32_t(A.bucket_slots)));
   d.valid = 1;
   return d;
 }
@@ -187,34 +203,31 @@ __device__ __forceinline__ void raw_desc_load(const JoinArgs& A, uint64_t item, 
 }
 
 // Decoded descriptor, moved to scalar registers (every field is uniform).
+// The raw words are read on every path (selects, not branches): a load whose
+// result some path leaves unread stays "pending" for the compiler, which then
+// waits for all memory operations before it reuses those registers -- at the
+// top of the next item, where that wait would serialise the prefetch.
 __device__ __forceinline__ ItemDesc desc_from_raw(const JoinArgs& A, const uint4& a, const uint4& b, uint64_t item,
                                                   uint64_t total) {
+  const bool valid = item < total;
   ItemDesc d;
-  if (item >= total) {
-    d.valid = 0;
-    d.q = d.rlo = d.rhi = d.slo = d.shi = 0;
-    d.fd = FastDiv{2, 0, -1, 0};
-    return d;
-  }
-  if (item >= A.desc_cap) {  // beyond the expanded descriptors
-    d = decode_item(A, item);
-  } else {
-    d.q = a.x;
-    d.rlo = a.y;
-    d.rhi = a.z;
-    d.slo = a.w;
-    d.shi = b.x;
-    d.fd.d = int32_t(join_table_slots(d.rhi - d.rlo, uint32_t(A.table_cap)));
-    d.fd.magic = int32_t(b.y);
-    d.fd.shift = int32_t(b.z);
-    d.fd.add_sign = int32_t(b.w);
-    d.valid = 1;
-  }
+  d.q = valid ? a.x : 0u;
+  d.rlo = valid ? a.y : 0u;
+  d.rhi = valid ? a.z : 0u;
+  d.slo = valid ? a.w : 0u;
+  d.shi = valid ? b.x : 0u;
+  d.fd.magic = valid ? int32_t(b.y) : 0;
+  d.fd.shift = valid ? int32_t(b.z) : -1;
+  d.fd.add_sign = valid ? int32_t(b.w) : 0;
+  d.fd.d = valid ? int32_t(join_table_slots(d.rhi - d.rlo, uint32_t(A.table_cap)) / uint32_t(A.bucket_slots)) : 2;
+  d.valid = valid ? 1u : 0u;
+  if (valid && item >= A.desc_cap) d = decode_item(A, item);  // beyond the expanded descriptors
   d.q = uniform(d.q);
   d.rlo = uniform(d.rlo);
   d.rhi = uniform(d.rhi);
   d.slo = uniform(d.slo);
   d.shi = uniform(d.shi);
+  d.valid = uniform(d.valid);
   d.fd.d = int32_t(uniform(uint32_t(d.fd.d)));
   d.fd.magic = int32_t(uniform(uint32_t(d.fd.magic)));
   d.fd.shift = int32_t(uniform(uint32_t(d.fd.shift)));
@@ -241,10 +254,12 @@ __global__ __launch_bounds__(BLOCK) void k_join(JoinArgs A) {
   constexpr uint32_t ROUND = uint32_t(R) * BLOCK;
   constexpr uint32_t STAGE = ROUND;
   constexpr int CNT_BITS = R < 2 ? 1 : (R < 4 ? 2 : (R < 8 ? 3 : (R < 16 ? 4 : 5)));  // bits of a count <= R
+  constexpr uint32_t BS = 16 / sizeof(key_t);  // slots per bucket
   static_assert(STAGE >= uint32_t(R) * kWave, "one wave's matches of a step must fit an empty stage");
   extern __shared__ __align__(16) uint8_t s_raw[];
-  tup_t* s_table = reinterpret_cast<tup_t*>(s_raw);                                     // table_cap slots
-  uint2* s_stage = reinterpret_cast<uint2*>(s_raw + size_t(A.table_cap) * sizeof(tup_t));  // STAGE pairs
+  key_t* s_keys = reinterpret_cast<key_t*>(s_raw);                                       // table_cap keys
+  int32_t* s_pays = reinterpret_cast<int32_t*>(s_raw + size_t(A.table_cap) * sizeof(key_t));  // their payloads
+  uint2* s_stage = reinterpret_cast<uint2*>(s_raw + size_t(A.table_cap) * (sizeof(key_t) + 4));  // STAGE pairs
   __shared__ StageCtl ctl;
   __shared__ uint32_t s_any[3];
   uint32_t vote = 0;
@@ -291,10 +306,13 @@ __global__ __launch_bounds__(BLOCK) void k_join(JoinArgs A) {
     if (dn.valid) load_item(dn, bn, pn);
     const uint64_t item_nn = item + 2 * uint64_t(gridDim.x);
     raw_desc_load(A, item_nn, ra, rb);  // decoded at the end of this item
-    const uint32_t tsize = uint32_t(d.fd.d);
+    const uint32_t tsize = uint32_t(d.fd.d) * BS;
     const key_t sentinel = key_t(d.q ^ 1u);
-    const tup_t empty = KTr::make(sentinel, 0);
-    for (uint32_t i = tid; i < tsize; i += BLOCK) s_table[i] = empty;
+    {
+      const uint4 fill = bucket_fill(sentinel);
+      uint4* s4 = reinterpret_cast<uint4*>(s_keys);
+      for (uint32_t i = tid; i < tsize / BS; i += BLOCK) s4[i] = fill;
+    }
     lds_barrier();
     // The previous item's matches go out now, after this item's prefetch was
     // issued: the in-order memory counter that the next wait for the prefetch
@@ -312,18 +330,18 @@ __global__ __launch_bounds__(BLOCK) void k_join(JoinArgs A) {
       for (int k = 0; k < NB; ++k) {
         const bool v = d.rlo + uint32_t(k) * BLOCK + tid < d.rhi;
         pend |= v ? (1u << k) : 0u;
-        bslot[k] = v ? hash_slot(murmur3(KTr::key(bt[k])), d.fd) : 0u;
+        bslot[k] = v ? BS * hash_slot(murmur3(KTr::key(bt[k])), d.fd) : 0u;
       }
       while (pend) {
         key_t old[NB];
 #pragma unroll
         for (int k = 0; k < NB; ++k)
-          if (pend & (1u << k)) old[k] = lds_cas(slot_key(&s_table[bslot[k]]), sentinel, KTr::key(bt[k]));
+          if (pend & (1u << k)) old[k] = lds_cas(&s_keys[bslot[k]], sentinel, KTr::key(bt[k]));
 #pragma unroll
         for (int k = 0; k < NB; ++k)
           if (pend & (1u << k)) {
             if (old[k] == sentinel) {
-              slot_set_pay(&s_table[bslot[k]], KTr::pay(bt[k]));
+              s_pays[bslot[k]] = KTr::pay(bt[k]);
               pend &= ~(1u << k);
             } else {
               dup |= old[k] == KTr::key(bt[k]);
@@ -339,8 +357,9 @@ __global__ __launch_bounds__(BLOCK) void k_join(JoinArgs A) {
     const bool first_match_only = UNIQUE || !chunk_has_dup;
 
     // A probe round of R tuples per thread with at most one match per probe:
-    // walk to the match or an empty slot, then append all matches at once
-    // (the stage is empty and holds R per thread).  Ends with a barrier.
+    // each step reads one bucket per unfinished probe, which ends at a match
+    // or at an empty slot; then all matches are appended at once (the stage
+    // is empty and holds R per thread).  Ends with a barrier.
     auto round_single = [&](const tup_t (&ptr)[R], uint32_t r0) {
       uint32_t slot[R];
       uint32_t walk = 0, found = 0;
@@ -348,23 +367,29 @@ __global__ __launch_bounds__(BLOCK) void k_join(JoinArgs A) {
       for (int k = 0; k < R; ++k) {
         const bool v = r0 + uint32_t(k) * BLOCK + tid < d.shi;
         walk |= v ? (1u << k) : 0u;
-        slot[k] = v ? hash_slot(murmur3(KTr::key(ptr[k])), d.fd) : 0u;
+        slot[k] = v ? BS * hash_slot(murmur3(KTr::key(ptr[k])), d.fd) : 0u;
       }
       while (__ballot(walk != 0)) {
-        key_t e[R];
-#pragma unroll
-        for (int k = 0; k < R; ++k)
-          if (walk & (1u << k)) e[k] = *slot_key(&s_table[slot[k]]);
 #pragma unroll
         for (int k = 0; k < R; ++k)
           if (walk & (1u << k)) {
-            if (e[k] == KTr::key(ptr[k])) {
+            key_t kk[BS];
+            bucket_keys(*reinterpret_cast<const uint4*>(&s_keys[slot[k]]), kk);
+            const key_t key = KTr::key(ptr[k]);
+            uint32_t hit = BS, empty = 0;
+#pragma unroll
+            for (int i = BS - 1; i >= 0; --i) {
+              hit = kk[i] == key ? uint32_t(i) : hit;
+              empty |= kk[i] == sentinel ? 1u : 0u;
+            }
+            if (hit < BS) {
               found |= 1u << k;
               walk &= ~(1u << k);
-            } else if (e[k] == sentinel) {
+              slot[k] += hit;
+            } else if (empty) {
               walk &= ~(1u << k);
             } else {
-              slot[k] = slot[k] + 1 == tsize ? 0u : slot[k] + 1;
+              slot[k] = slot[k] + BS == tsize ? 0u : slot[k] + BS;
             }
           }
       }
@@ -383,7 +408,7 @@ __global__ __launch_bounds__(BLOCK) void k_join(JoinArgs A) {
 #pragma unroll
         for (int k = 0; k < R; ++k)
           if ((found & (1u << k)) && pos < STAGE)
-            s_stage[pos++] = make_uint2(uint32_t(KTr::pay(s_table[slot[k]])), uint32_t(KTr::pay(ptr[k])));
+            s_stage[pos++] = make_uint2(uint32_t(s_pays[slot[k]]), uint32_t(KTr::pay(ptr[k])));
       }
       lds_barrier();
     };
@@ -400,14 +425,14 @@ __global__ __launch_bounds__(BLOCK) void k_join(JoinArgs A) {
       for (int k = 0; k < R; ++k) {
         const bool v = r0 + uint32_t(k) * BLOCK + tid < d.shi;
         walk |= v ? (1u << k) : 0u;
-        slot[k] = v ? hash_slot(murmur3(KTr::key(ptr[k])), d.fd) : 0u;
+        slot[k] = v ? BS * hash_slot(murmur3(KTr::key(ptr[k])), d.fd) : 0u;
       }
       while (true) {
         while (__ballot(walk != 0)) {
           key_t e[R];
 #pragma unroll
           for (int k = 0; k < R; ++k)
-            if (walk & (1u << k)) e[k] = *slot_key(&s_table[slot[k]]);
+            if (walk & (1u << k)) e[k] = s_keys[slot[k]];
           uint32_t hit = 0;
 #pragma unroll
           for (int k = 0; k < R; ++k)
@@ -434,7 +459,7 @@ __global__ __launch_bounds__(BLOCK) void k_join(JoinArgs A) {
             for (int k = 0; k < R; ++k)
               if (hit & (1u << k)) {
                 if (pos < STAGE) {
-                  s_stage[pos] = make_uint2(uint32_t(KTr::pay(s_table[slot[k]])), uint32_t(KTr::pay(ptr[k])));
+                  s_stage[pos] = make_uint2(uint32_t(s_pays[slot[k]]), uint32_t(KTr::pay(ptr[k])));
                   slot[k] = slot[k] + 1 == tsize ? 0u : slot[k] + 1;
                 }
                 ++pos;  // a match past the stage stays pending at its slot
@@ -607,11 +632,9 @@ static hipError_t launch_join_kernel(const JoinArgs& A, int key_bytes, int base,
   const size_t lds = join_lds_bytes(key_bytes, A.table_cap);
   // build tuples per thread: NB * kJoinBlock >= base (base <= table_cap / 2 - 1)
   if (key_bytes == 4) {
-    static_assert(6 * kJoinBlock >= (kJoinLdsBytes - kJoinProbe4 * kJoinBlock * 8) / 16, "NB = 6 covers the cap");
     if (base <= 3 * kJoinBlock) join_launch_t<4, 3, kJoinProbe4>(A, unique, lds, s);
     else join_launch_t<4, 6, kJoinProbe4>(A, unique, lds, s);
   } else {
-    static_assert(4 * kJoinBlock >= (kJoinLdsBytes - kJoinProbe8 * kJoinBlock * 8) / 32, "NB = 4 covers the cap");
     if (base <= 2 * kJoinBlock) join_launch_t<8, 2, kJoinProbe8>(A, unique, lds, s);
     else join_launch_t<8, 4, kJoinProbe8>(A, unique, lds, s);
   }
@@ -639,6 +662,7 @@ static JoinArgs join_args(const Layout& L, int key_bytes, int base, int64_t slic
   A.desc = reinterpret_cast<const uint4*>(L.desc);
   A.desc_cap = L.desc_cap;
   A.table_cap = join_table_capacity(key_bytes);
+  A.bucket_slots = 16 / key_bytes;
   return A;
 }
 

@@ -24,6 +24,7 @@
 //              assignment, src/skewed_data_handler.cuh:58-91).
 // Segment s of a table is processed on XCD s / (nseg/8) by both the histogram
 // and pass 1 (workgroup b runs on XCD b % 8).
+#include <cmath>
 #include <type_traits>
 
 #include "pj_device.h"
@@ -710,13 +711,20 @@ hipError_t launch_pass1(const Layout& L, int key_bytes, hipStream_t s) {
 // k_pass2: items (p1, segment group) -> split by digit key & (F2 - 1).
 // ---------------------------------------------------------------------------
 template <int KB, int BLOCK, int TILE>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void k_pass2(PartTable t0, PartTable t1, PartParams pp) {
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void k_pass2(PartTable t0, PartTable t1,
+                                                                                        PartParams pp, uint32_t g0) {
   using KTr = KeyT<KB>;
   using key_t = typename KTr::key_t;
   using tup_t = typename KTr::tup_t;
   constexpr int E = TILE / BLOCK;
-  const PartTable T = blockIdx.y ? t1 : t0;
-  const int x = blockIdx.x % kNumXcd, j = blockIdx.x / kNumXcd, G = gridDim.x / kNumXcd;
+  // Workgroups [0, g0) serve table 0 and [g0, grid) table 1, in proportion
+  // to the tables' sizes; g0 is a multiple of 8, so a workgroup's XCD is its
+  // index within its table modulo 8.
+  const bool second = blockIdx.x >= g0;
+  const PartTable T = second ? t1 : t0;
+  const uint32_t local = second ? blockIdx.x - g0 : blockIdx.x;
+  const uint32_t groups = second ? gridDim.x - g0 : g0;
+  const int x = int(local % kNumXcd), j = int(local / kNumXcd), G = int(groups / kNumXcd);
   const int F1 = pp.F1, F2 = pp.F2, b2 = pp.b2, P = pp.P;
   const uint32_t mask = uint32_t(F2 - 1);
   const uint32_t nseg = T.nseg;
@@ -767,14 +775,16 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
         for (int e = 0; e < E; ++e) dg[e] = digit_of(KTr::low32(KTr::key(cur[e])), pp.shift0, mask);
         rank_tile<E>(dg, s_cnt, pk);
       }
-      const uint32_t next_lo = in_lo + (t + 1 < nfull ? t + 1 : t) * uint32_t(TILE);
+      // Next tile in flight: the next full tile, or after the last one the
+      // item's partial tail (indices clamped to the item, so the loads need
+      // no condition and nothing is read twice).
+      const uint32_t next_lo = in_lo + (t + 1) * uint32_t(TILE);
 #pragma unroll
-      for (int e = 0; e < E; ++e) nxt[e] = in[next_lo + uint32_t(e) * BLOCK + tid];  // next tile in flight
+      for (int e = 0; e < E; ++e) nxt[e] = in[min(next_lo + uint32_t(e) * BLOCK + tid, in_hi - 1)];
       scan_stage_tile<KB, BLOCK, E>(cur, pk, F2, s_stage, s_cnt, s_base, s_cur, s_delta, s_tmp);
     };
-    tup_t tv[E];
+    tup_t tv[E], tw[E];
     if (nfull > 0) {
-      tup_t tw[E];
 #pragma unroll
       for (int e = 0; e < E; ++e) tv[e] = in[in_lo + uint32_t(e) * BLOCK + tid];
       full_tile(0u, std::false_type{}, tv, tw);
@@ -785,14 +795,21 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
       }
       if (t < nfull) full_tile(t, std::true_type{}, tw, tv);
       flush_full<KB, BLOCK, E>(pp.shift0, mask, s_stage, s_delta, out, T.n);
+      // the prefetched tail is in tw after an odd number of full tiles
+      if (nfull & 1u) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) tv[e] = tw[e];
+      }
     }
     // The partial last tile of the item.
     const uint32_t tail_lo = in_lo + nfull * uint32_t(TILE);
     if (tail_lo < in_hi) {
+      if (nfull == 0) {
 #pragma unroll
-      for (int e = 0; e < E; ++e) {
-        const uint32_t i = tail_lo + uint32_t(e) * BLOCK + tid;
-        if (i < in_hi) tv[e] = in[i];  // lanes past the end are marked empty by their digit
+        for (int e = 0; e < E; ++e) {
+          const uint32_t i = tail_lo + uint32_t(e) * BLOCK + tid;
+          if (i < in_hi) tv[e] = in[i];  // lanes past the end are marked empty by their digit
+        }
       }
       for (int d = tid; d < F2; d += BLOCK) s_cnt[d] = 0;
       lds_barrier();
@@ -806,13 +823,25 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
 }
 
 hipError_t launch_pass2(const Layout& L, int key_bytes, hipStream_t s) {
-  dim3 grid(kNumXcd * kPass2WgPerXcd, L.ntables), block(kScatterBlock);
+  // Persistent workgroups, kPass2WgPerXcd per XCD and table on average, split
+  // between the tables by their sizes (the work is proportional to them).
+  const uint32_t per_xcd = uint32_t(kPass2WgPerXcd) * uint32_t(L.ntables);
+  uint32_t x0 = per_xcd;
+  if (L.ntables > 1) {
+    const double n0 = double(L.t[0].n), n1 = double(L.t[1].n);
+    const double share = n0 + n1 > 0 ? n0 / (n0 + n1) : 0.5;
+    x0 = uint32_t(std::lround(share * per_xcd));
+    if (x0 < 1) x0 = 1;
+    if (x0 > per_xcd - 1) x0 = per_xcd - 1;
+  }
+  const uint32_t g0 = x0 * uint32_t(kNumXcd);
+  dim3 grid(per_xcd * kNumXcd), block(kScatterBlock);
   if (key_bytes == 4) {
     const size_t lds = scatter_lds_bytes<4, kScatterTile4>(L.pp.F2);
-    hipLaunchKernelGGL((k_pass2<4, kScatterBlock, kScatterTile4>), grid, block, lds, s, L.t[0], L.t[1], L.pp);
+    hipLaunchKernelGGL((k_pass2<4, kScatterBlock, kScatterTile4>), grid, block, lds, s, L.t[0], L.t[1], L.pp, g0);
   } else {
     const size_t lds = scatter_lds_bytes<8, kScatterTile8>(L.pp.F2);
-    hipLaunchKernelGGL((k_pass2<8, kScatterBlock, kScatterTile8>), grid, block, lds, s, L.t[0], L.t[1], L.pp);
+    hipLaunchKernelGGL((k_pass2<8, kScatterBlock, kScatterTile8>), grid, block, lds, s, L.t[0], L.t[1], L.pp, g0);
   }
   return hipGetLastError();
 }

@@ -35,7 +35,7 @@ def _cands(n_valid=10, heavy=(), n_probe=1 << 22):
 def test_shapes_and_capacity(pj):
     from partitioned_join_amd import planner
     assert planner.param_count() == 1030 * 256 + 256 + 256 * 256 + 256 + 256 * 81 + 81
-    assert planner.chunk_capacity(4) == 2783 and planner.chunk_capacity(8) == 2031
+    assert planner.chunk_capacity(4) == 2783 and planner.chunk_capacity(8) == 2047
     assert planner.chunk_capacity(3) == 0
 
 
