@@ -27,13 +27,20 @@ template <> struct KeyT<4> {
   __device__ __forceinline__ static uint32_t low32(key_t k) { return k; }
 };
 
+// A plain 16-byte aggregate rather than HIP's uint4 (a union-based vector
+// type): arrays of it held in registers are split into scalars by the
+// compiler, where arrays of uint4 can end up in scratch memory.
+struct alignas(16) Tup8 {
+  uint32_t x, y, z, w;
+};
+
 template <> struct KeyT<8> {
   using key_t = uint64_t;
-  using tup_t = uint4;
+  using tup_t = Tup8;
   __device__ __forceinline__ static key_t key(const tup_t& t) { return uint64_t(t.x) | (uint64_t(t.y) << 32); }
   __device__ __forceinline__ static int32_t pay(const tup_t& t) { return int32_t(t.z); }
   __device__ __forceinline__ static tup_t make(key_t k, int32_t p) {
-    return make_uint4(uint32_t(k), uint32_t(k >> 32), uint32_t(p), 0u);
+    return Tup8{uint32_t(k), uint32_t(k >> 32), uint32_t(p), 0u};
   }
   __device__ __forceinline__ static uint32_t low32(key_t k) { return uint32_t(k); }
 };

@@ -11,16 +11,16 @@ namespace pj {
 
 // Tuning constants (gfx950: 256 CUs, 160 KB LDS per CU, wave64).
 constexpr int kHistBlock = 1024;      // combined histogram workgroup
-constexpr int kScatterBlock = 512;    // partition pass workgroup
-constexpr int kScatterTile4 = 8192;   // tuples staged in LDS per tile, 4-byte keys (64 KB)
-constexpr int kScatterTile8 = 4096;   // 8-byte keys (64 KB)
+constexpr int kScatterBlock = 1024;   // partition pass workgroup (one per CU: 16 waves)
+constexpr int kScatterTile4 = 16384;  // tuples staged in LDS per tile, 4-byte keys (128 KB)
+constexpr int kScatterTile8 = 8192;   // 8-byte keys (128 KB)
 constexpr int kJoinBlock = 512;       // join workgroup
 constexpr int kJoinProbe4 = 9;        // probe tuples per thread and round, 4-byte keys (4608 per round)
 constexpr int kJoinProbe8 = 4;        // 8-byte keys (2048 per round)
 constexpr int kJoinLdsBytes = 81408;  // dynamic LDS per join workgroup: two workgroups share a CU's 160 KB
 constexpr int64_t kOutChunk = 16384;  // output pairs claimed per global allocation
 constexpr int kJoinGrid = 512;        // persistent join workgroups (2 per CU)
-constexpr int kPass2WgPerXcd = 32;    // pass-2 persistent workgroups per XCD and table
+constexpr int kPass2WgPerXcd = 16;    // pass-2 persistent workgroups per XCD and table (one per CU)
 constexpr int kPass2Groups = 16;      // segment groups per pass-1 partition in pass 2
 constexpr int kMaxTotalBits = 16;     // combined histogram limit (65536 partitions)
 constexpr int kMaxSegPerXcd = 64;

@@ -432,9 +432,11 @@ hipError_t launch_join_plan(const Layout& L, int base, int64_t slice, hipStream_
 // ranking and the scan, when the fewest registers are live.  Tile t is
 // flushed at the start of tile t + 1, before the loads of tile t + 2: the
 // memory counter completes in order, so the wait for those loads finds the
-// stores long complete instead of waiting for them at once.  The barrier that
-// follows the clearing of s_cnt for tile t + 1 orders the flush's LDS reads
-// before tile t + 1 rewrites s_delta and the stage.
+// stores long complete instead of waiting for them at once.  Two count arrays
+// alternate between tiles: the one for tile t + 1 is cleared while tile t is
+// staged, so no barrier of its own is needed; the first barrier of tile t + 1
+// (after its ranking) orders the flush's LDS reads before the scan rewrites
+// s_delta and the stage.
 // ---------------------------------------------------------------------------
 // Ranks of a tile's items within their digits, from LDS atomics on the
 // cleared s_cnt: pk = digit << 16 | rank (digits < 4096, ranks < 65536), or
@@ -446,35 +448,44 @@ __device__ __forceinline__ void rank_tile(const uint32_t (&dg)[E], uint32_t* s_c
     pk[e] = dg[e] != 0xFFFFFFFFu ? (dg[e] << 16) | atomicAdd(&s_cnt[dg[e]], 1u) : 0xFFFFFFFFu;
 }
 
-// After rank_tile: scans the tile histogram, advances the per-digit global
-// cursors s_cur, records s_delta (global minus staged position per digit) and
-// stages the tile in digit order.  Three barriers; ends with one.
+// After rank_tile: the first wave scans the F digit counts (F / 64 per lane,
+// no barrier), advances the per-digit global cursors s_cur and records s_delta
+// (global minus staged position per digit); then the tile is staged in digit
+// order while s_cnt_next is cleared for the next tile.  Three barriers; ends
+// with one.
 template <int KB, int BLOCK, int E>
 __device__ __forceinline__ void scan_stage_tile(const typename KeyT<KB>::tup_t (&tup)[E], const uint32_t (&pk)[E],
                                                 int F, typename KeyT<KB>::tup_t* s_stage, const uint32_t* s_cnt,
-                                                uint32_t* s_base, uint32_t* s_cur, uint32_t* s_delta,
-                                                uint32_t* s_tmp) {
+                                                uint32_t* s_cnt_next, uint32_t* s_base, uint32_t* s_cur,
+                                                uint32_t* s_delta) {
   const int tid = threadIdx.x;
   lds_barrier();
-  // Exclusive scan of the F digit counts: each thread owns DPT consecutive digits.
-  const int DPT = F > BLOCK ? F / BLOCK : 1;
-  const int d0 = tid * DPT;
-  uint32_t local = 0;
-  for (int k = 0; k < DPT; ++k)
-    if (d0 + k < F) local += s_cnt[d0 + k];
-  uint32_t total;
-  uint32_t run = block_exclusive_scan<BLOCK>(local, s_tmp, &total);
-  for (int k = 0; k < DPT; ++k) {
-    const int d = d0 + k;
-    if (d < F) {
-      const uint32_t c = s_cnt[d];
-      s_base[d] = run;
-      s_delta[d] = s_cur[d] - run;
-      s_cur[d] += c;
-      run += c;
+  if (tid < kWave) {
+    const int DPL = F > kWave ? F / kWave : 1;  // digits per lane
+    const int d0 = tid * DPL;
+    uint32_t local = 0;
+    for (int k = 0; k < DPL; ++k)
+      if (d0 + k < F) local += s_cnt[d0 + k];
+    uint32_t incl = local;
+#pragma unroll
+    for (int dlt = 1; dlt < kWave; dlt <<= 1) {
+      const uint32_t y = __shfl_up(incl, dlt, kWave);
+      if (tid >= dlt) incl += y;
+    }
+    uint32_t run = incl - local;
+    for (int k = 0; k < DPL; ++k) {
+      const int d = d0 + k;
+      if (d < F) {
+        const uint32_t c = s_cnt[d];
+        s_base[d] = run;
+        s_delta[d] = s_cur[d] - run;
+        s_cur[d] += c;
+        run += c;
+      }
     }
   }
   lds_barrier();
+  for (int d = tid; d < F; d += BLOCK) s_cnt_next[d] = 0;
 #pragma unroll
   for (int e = 0; e < E; ++e)
     if (pk[e] != 0xFFFFFFFFu) s_stage[s_base[pk[e] >> 16] + (pk[e] & 0xFFFFu)] = tup[e];
@@ -513,11 +524,17 @@ __device__ __forceinline__ void flush_tile(uint32_t len, int shift, uint32_t mas
   }
 }
 
-// LDS layout of the scatter kernels: stage[TILE] then cnt, base, cur, delta [F] and tmp.
+// LDS layout of the scatter kernels: stage[TILE], then two count arrays,
+// base, cur and delta, F words each.
 template <int KB, int TILE>
 __host__ __device__ constexpr size_t scatter_lds_bytes(int F) {
-  return size_t(TILE) * (KB == 4 ? 8 : 16) + size_t(F) * 16 + 64 * 4;
+  return size_t(TILE) * (KB == 4 ? 8 : 16) + size_t(F) * 20;
 }
+
+// Tile of a scatter pass: the large tile while the count arrays fit beside
+// it (F <= 1024), else half of it (128 KB + 20 KB or 64 KB + 80 KB of LDS).
+template <int KB>
+__host__ __device__ constexpr int scatter_tile_big() { return KB == 4 ? kScatterTile4 : kScatterTile8; }
 
 // ---------------------------------------------------------------------------
 // k_pass1: one workgroup per segment; digit (key >> b2) & (F1 - 1).  Full
@@ -541,15 +558,24 @@ __device__ __forceinline__ void pass1_body(const PartTable& T, const PartParams&
 
   extern __shared__ __align__(16) uint8_t s_raw[];
   tup_t* s_stage = reinterpret_cast<tup_t*>(s_raw);
-  uint32_t* s_cnt = reinterpret_cast<uint32_t*>(s_raw + size_t(TILE) * sizeof(tup_t));
-  uint32_t* s_base = s_cnt + F;
+  uint32_t* s_cnt = reinterpret_cast<uint32_t*>(s_raw + size_t(TILE) * sizeof(tup_t));  // this tile's counts
+  uint32_t* s_cnt_next = s_cnt + F;                                                     // the next tile's
+  uint32_t* s_base = s_cnt_next + F;
   uint32_t* s_cur = s_base + F;
   uint32_t* s_delta = s_cur + F;
-  uint32_t* s_tmp = s_delta + F;
   const int tid = threadIdx.x;
 
   const uint32_t* rel = (pp.passes == 2) ? T.matc + size_t(seg) * F : T.mat + size_t(seg) * F;
-  for (int d = tid; d < F; d += BLOCK) s_cur[d] = T.off[size_t(d) << pp.b2] + rel[d];
+  for (int d = tid; d < F; d += BLOCK) {
+    s_cur[d] = T.off[size_t(d) << pp.b2] + rel[d];
+    s_cnt[d] = 0;
+  }
+  lds_barrier();
+  auto swap_counts = [&]() {
+    uint32_t* t = s_cnt;
+    s_cnt = s_cnt_next;
+    s_cnt_next = t;
+  };
   tup_t* out = static_cast<tup_t*>(T.buf1);
   const key_t* keys = static_cast<const key_t*>(T.keys);
 
@@ -620,8 +646,6 @@ __device__ __forceinline__ void pass1_body(const PartTable& T, const PartParams&
   auto full_tile = [&](uint32_t t, auto flush_prev, const key_t (&ck)[E], const int32_t (&cp)[E], key_t (&nk)[E],
                        int32_t (&np)[E]) {
     if (decltype(flush_prev)::value) flush_full<KB, BLOCK, E>(shift, mask, s_stage, s_delta, out, T.n);
-    for (int d = tid; d < F; d += BLOCK) s_cnt[d] = 0;
-    lds_barrier();
     uint32_t pk[E];
     {
       uint32_t dg[E];
@@ -636,7 +660,8 @@ __device__ __forceinline__ void pass1_body(const PartTable& T, const PartParams&
 #pragma unroll
     for (int e = 0; e < E; ++e)
       tup[e] = KTr::make(ck[e], PAY ? cp[e] : int32_t(tile_lo + (uint32_t(e / 4) * BLOCK + tid) * 4 + (e % 4)));
-    scan_stage_tile<KB, BLOCK, E>(tup, pk, F, s_stage, s_cnt, s_base, s_cur, s_delta, s_tmp);
+    scan_stage_tile<KB, BLOCK, E>(tup, pk, F, s_stage, s_cnt, s_cnt_next, s_base, s_cur, s_delta);
+    swap_counts();
   };
   key_t key[E];
   int32_t pay[E];
@@ -657,15 +682,13 @@ __device__ __forceinline__ void pass1_body(const PartTable& T, const PartParams&
   const uint32_t tail_lo = lo + nfull * uint32_t(TILE);
   if (tail_lo < hi) {
     load_tail(tail_lo, key, pay);
-    for (int d = tid; d < F; d += BLOCK) s_cnt[d] = 0;
-    lds_barrier();
     uint32_t dg[E], pk[E];
     digits(tail_lo, key, dg);
     rank_tile<E>(dg, s_cnt, pk);
     tup_t tup[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) tup[e] = KTr::make(key[e], pay[e]);
-    scan_stage_tile<KB, BLOCK, E>(tup, pk, F, s_stage, s_cnt, s_base, s_cur, s_delta, s_tmp);
+    scan_stage_tile<KB, BLOCK, E>(tup, pk, F, s_stage, s_cnt, s_cnt_next, s_base, s_cur, s_delta);
     flush_tile<KB, BLOCK, E>(hi - tail_lo, shift, mask, s_stage, s_delta, out, T.n);
   }
 }
@@ -680,14 +703,20 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
   }
 }
 
-template <int KB, bool VEC, bool PAY0, bool PAY1>
-static hipError_t pass1_launch(const Layout& L, hipStream_t s) {
-  constexpr int TILE = KB == 4 ? kScatterTile4 : kScatterTile8;
+template <int KB, bool VEC, bool PAY0, bool PAY1, int TILE>
+static hipError_t pass1_launch_t(const Layout& L, hipStream_t s) {
   const uint32_t nseg = L.t[0].nseg > L.t[1].nseg ? L.t[0].nseg : L.t[1].nseg;
   const size_t lds = scatter_lds_bytes<KB, TILE>(L.pp.F1);
   hipLaunchKernelGGL((k_pass1<KB, VEC, PAY0, PAY1, kScatterBlock, TILE>), dim3(nseg, L.ntables),
                      dim3(kScatterBlock), lds, s, L.t[0], L.t[1], L.pp);
   return hipGetLastError();
+}
+
+template <int KB, bool VEC, bool PAY0, bool PAY1>
+static hipError_t pass1_launch(const Layout& L, hipStream_t s) {
+  constexpr int BIG = scatter_tile_big<KB>();
+  if (L.pp.F1 <= 1024) return pass1_launch_t<KB, VEC, PAY0, PAY1, BIG>(L, s);
+  return pass1_launch_t<KB, VEC, PAY0, PAY1, BIG / 2>(L, s);
 }
 
 template <int KB, bool VEC>
@@ -734,12 +763,19 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
 
   extern __shared__ __align__(16) uint8_t s_raw[];
   tup_t* s_stage = reinterpret_cast<tup_t*>(s_raw);
-  uint32_t* s_cnt = reinterpret_cast<uint32_t*>(s_raw + size_t(TILE) * sizeof(tup_t));
-  uint32_t* s_base = s_cnt + F2;
+  uint32_t* s_cnt = reinterpret_cast<uint32_t*>(s_raw + size_t(TILE) * sizeof(tup_t));  // this tile's counts
+  uint32_t* s_cnt_next = s_cnt + F2;                                                    // the next tile's
+  uint32_t* s_base = s_cnt_next + F2;
   uint32_t* s_cur = s_base + F2;
   uint32_t* s_delta = s_cur + F2;
-  uint32_t* s_tmp = s_delta + F2;
   const int tid = threadIdx.x;
+  for (int d = tid; d < F2; d += BLOCK) s_cnt[d] = 0;
+  lds_barrier();
+  auto swap_counts = [&]() {
+    uint32_t* t = s_cnt;
+    s_cnt = s_cnt_next;
+    s_cnt_next = t;
+  };
   const tup_t* in = static_cast<const tup_t*>(T.buf1);
   tup_t* out = static_cast<tup_t*>(T.buf2);
 
@@ -766,8 +802,6 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
     const uint32_t nfull = (in_hi - in_lo) / uint32_t(TILE);
     auto full_tile = [&](uint32_t t, auto flush_prev, const tup_t (&cur)[E], tup_t (&nxt)[E]) {
       if (decltype(flush_prev)::value) flush_full<KB, BLOCK, E>(pp.shift0, mask, s_stage, s_delta, out, T.n);
-      for (int d = tid; d < F2; d += BLOCK) s_cnt[d] = 0;
-      lds_barrier();
       uint32_t pk[E];
       {
         uint32_t dg[E];
@@ -781,7 +815,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
       const uint32_t next_lo = in_lo + (t + 1) * uint32_t(TILE);
 #pragma unroll
       for (int e = 0; e < E; ++e) nxt[e] = in[min(next_lo + uint32_t(e) * BLOCK + tid, in_hi - 1)];
-      scan_stage_tile<KB, BLOCK, E>(cur, pk, F2, s_stage, s_cnt, s_base, s_cur, s_delta, s_tmp);
+      scan_stage_tile<KB, BLOCK, E>(cur, pk, F2, s_stage, s_cnt, s_cnt_next, s_base, s_cur, s_delta);
+      swap_counts();
     };
     tup_t tv[E], tw[E];
     if (nfull > 0) {
@@ -811,12 +846,11 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
           if (i < in_hi) tv[e] = in[i];  // lanes past the end are marked empty by their digit
         }
       }
-      for (int d = tid; d < F2; d += BLOCK) s_cnt[d] = 0;
-      lds_barrier();
       uint32_t dg[E], pk[E];
       digits(tv, tail_lo, in_hi, dg);
       rank_tile<E>(dg, s_cnt, pk);
-      scan_stage_tile<KB, BLOCK, E>(tv, pk, F2, s_stage, s_cnt, s_base, s_cur, s_delta, s_tmp);
+      scan_stage_tile<KB, BLOCK, E>(tv, pk, F2, s_stage, s_cnt, s_cnt_next, s_base, s_cur, s_delta);
+      swap_counts();
       flush_tile<KB, BLOCK, E>(in_hi - tail_lo, pp.shift0, mask, s_stage, s_delta, out, T.n);
     }
   }
@@ -836,13 +870,18 @@ hipError_t launch_pass2(const Layout& L, int key_bytes, hipStream_t s) {
   }
   const uint32_t g0 = x0 * uint32_t(kNumXcd);
   dim3 grid(per_xcd * kNumXcd), block(kScatterBlock);
+  const bool big = L.pp.F2 <= 1024;
+#define PJ_PASS2(KB_, TILE_)                                                                                     \
+  hipLaunchKernelGGL((k_pass2<KB_, kScatterBlock, TILE_>), grid, block, (scatter_lds_bytes<KB_, TILE_>(L.pp.F2)), s, \
+                     L.t[0], L.t[1], L.pp, g0)
   if (key_bytes == 4) {
-    const size_t lds = scatter_lds_bytes<4, kScatterTile4>(L.pp.F2);
-    hipLaunchKernelGGL((k_pass2<4, kScatterBlock, kScatterTile4>), grid, block, lds, s, L.t[0], L.t[1], L.pp, g0);
+    if (big) PJ_PASS2(4, kScatterTile4);
+    else PJ_PASS2(4, kScatterTile4 / 2);
   } else {
-    const size_t lds = scatter_lds_bytes<8, kScatterTile8>(L.pp.F2);
-    hipLaunchKernelGGL((k_pass2<8, kScatterBlock, kScatterTile8>), grid, block, lds, s, L.t[0], L.t[1], L.pp, g0);
+    if (big) PJ_PASS2(8, kScatterTile8);
+    else PJ_PASS2(8, kScatterTile8 / 2);
   }
+#undef PJ_PASS2
   return hipGetLastError();
 }
 
