@@ -374,6 +374,7 @@ def main(argv: Optional[list] = None):
     b.add_argument("--data", nargs="+", required=True, help=".npz/.json prefixes from `label`")
     b.add_argument("--epochs", type=int, default=60)
     b.add_argument("--holdout", type=float, default=0.2)
+    b.add_argument("--weights", default=WEIGHTS_PATH, help="where to write the trained weights")
     args = ap.parse_args(argv)
     if args.cmd == "label":
         build_dataset(args.first, args.count, args.out, log=lambda s: print(s, flush=True))
@@ -393,9 +394,10 @@ def main(argv: Optional[list] = None):
     ev_tr = evaluate(w, X[tr], Y[tr], H[tr], C[tr], E[tr], recs[tr])
     ev_te = evaluate(w, X[te], Y[te], H[te], C[te], E[te], recs[te])
     print(json.dumps({"train": ev_tr, "holdout": ev_te}), flush=True)
+    meta_path = os.path.splitext(args.weights)[0] + ".json"
     save_weights(w, {"features": FEATURES, "hidden": HIDDEN, "logits": LOGITS, "train": ev_tr, "holdout": ev_te,
                      "joins": n, "epochs": args.epochs,
-                     "data": "seeded training joins i = 0..n-1 of planner.join_spec"})
+                     "data": "seeded training joins i = 0..n-1 of planner.join_spec"}, args.weights, meta_path)
 
 
 if __name__ == "__main__":
