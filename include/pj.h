@@ -309,10 +309,12 @@ pj_status pj_planner_train_step(float* d_weights, float* d_adam_m, float* d_adam
                                 const float* d_features, const int32_t* d_labels, const float* d_heavy,
                                 int32_t batch, float* d_loss, void* d_scratch, size_t scratch_bytes,
                                 hipStream_t stream);
-/* Host only: the plan from one sample's logits.  The predicted heavy set (flag
- * logit > 0) must equal the exact one (count * 1024 >= |S|) on every candidate
- * and the plan's build chunks must fit the LDS table; otherwise the plain
- * plan is returned and *fallback = 1. */
+/* Host only: the plan from one sample's logits.  A key is heavy with at least
+ * T = |S| / 1024 probe rows.  The predicted heavy set (flag logit > 0)
+ * disagrees with the exact counts if a candidate predicted heavy has fewer
+ * than T / 2 rows or one not predicted heavy has 2T or more.  On any
+ * disagreement, or if the plan's build chunks would not fit the LDS table,
+ * the plain plan is returned and *fallback = 1. */
 pj_status pj_plan_decode(const float* logits, const uint32_t* cand_sample_counts, const uint64_t* cand_exact_counts,
                          const pj_table* build, const pj_table* probe, pj_plan* plan, int32_t* fallback);
 size_t pj_plan_learned_workspace_size(void);

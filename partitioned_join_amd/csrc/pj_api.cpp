@@ -806,15 +806,20 @@ pj_status pj_plan_decode(const float* logits, const uint32_t* cand_sample_counts
       if (logits[off + c] > logits[off + best]) best = c;
     return best;
   };
-  // Heavy key: at least 1/1024 of the probe rows.  The predicted set must
-  // equal the exact one on every valid candidate, else the plain plan runs.
+  // Heavy key: at least T = |S| / 1024 probe rows.  The predicted set
+  // disagrees with the exact counts when a key predicted heavy has fewer than
+  // T / 2 rows or a key not predicted heavy has 2T or more; keys within a
+  // factor of two of T (where a 4096-key sample cannot decide) count either
+  // way.  Any disagreement selects the plain plan.
   bool agree = true;
   const int hoff = 2 * kPlanBitsCls + kPlanPassCls;
+  const uint64_t ns = uint64_t(probe->n);
   for (int j = 0; j < kPlanCand; ++j) {
     if (cand_sample_counts[j] == 0) continue;
     const bool pred = logits[hoff + j] > 0.f;
-    const bool exact = probe->n > 0 && cand_exact_counts[j] * 1024ull >= uint64_t(probe->n);
-    agree = agree && pred == exact;
+    const uint64_t c = cand_exact_counts[j];
+    if (pred && c * 2048ull < ns) agree = false;         // predicted heavy, below T / 2
+    if (!pred && c * 512ull >= ns && ns > 0) agree = false;  // missed, at least 2T
   }
   pj_plan learned;
   plain_plan(build->n, build->key_bytes, &learned);

@@ -274,6 +274,16 @@ def time_plan(bk, pk, plan_tuple, reps: int = 2):
     return best, n
 
 
+def heavy_targets(cand_counts, exact_counts, n_probe: int):
+    """Training targets of the heavy flags, matching the library's agreement
+    rule (T = |S| / 1024): 1 for at least 2T rows, 0 below T / 2, -1 (no
+    loss) in between, where either answer is accepted, and for missing
+    candidates."""
+    c = np.asarray(exact_counts, dtype=np.float64)
+    t = np.where(c * 512 >= n_probe, 1.0, np.where(c * 2048 < n_probe, 0.0, -1.0))
+    return np.where(np.asarray(cand_counts) > 0, t, -1.0).astype(np.float32)
+
+
 def label_join(spec, device="cuda", reps: int = 2):
     """Features, candidate info, heavy targets and plan timings of one join."""
     torch = _t()
@@ -282,7 +292,7 @@ def label_join(spec, device="cuda", reps: int = 2):
     exact = candidate_counts(pk, cand, cnt)
     cnt_h = cnt.cpu().numpy()
     exact_h = exact.cpu().numpy()
-    heavy = np.where(cnt_h > 0, (exact_h * 1024 >= pk.numel()).astype(np.float32), -1.0).astype(np.float32)
+    heavy = heavy_targets(cnt_h, exact_h, pk.numel())
     timings = {}
     counts = set()
     for p in candidate_plans(bk.numel(), bk.element_size()):
@@ -386,6 +396,7 @@ def main(argv: Optional[list] = None):
         with open(pre + ".json") as fh:
             recs += json.load(fh)
     X, Y, H, C, E = (np.concatenate(t) for t in (X, Y, H, C, E))
+    H = np.stack([heavy_targets(C[i], E[i], recs[i]["spec"]["n_probe"]) for i in range(len(recs))])
     n = len(recs)
     n_test = int(n * args.holdout)
     tr = slice(0, n - n_test)

@@ -23,12 +23,15 @@ def _logits(b1=2, b2=2, passes=1, heavy=()):
     return lg
 
 
-def _cands(n_valid=10, heavy=(), n_probe=1 << 22):
+def _cands(n_valid=10, heavy=(), n_probe=1 << 22, border=()):
+    """Candidates with exact counts clearly heavy (2T, T = |S| / 1024), clearly
+    light (below T / 2) or, for `border`, T (where either prediction passes)."""
+    T = n_probe // 1024
     cc = np.zeros(64, dtype=np.uint32)
     cc[:n_valid] = 7
     ex = np.zeros(64, dtype=np.uint64)
     for j in range(n_valid):
-        ex[j] = (n_probe // 1024) if j in heavy else (n_probe // 1024 - 1)
+        ex[j] = 2 * T if j in heavy else (T if j in border else T // 2 - 1)
     return cc, ex
 
 
@@ -62,6 +65,32 @@ def test_decode_falls_back_on_any_heavy_disagreement(pj):
     cc2[3] = 0
     plan, fb = planner.decode(_logits(2, 2, 1, heavy=(0, 1, 2, 3)), cc2, ex, n_b, n_p, 4)
     assert not fb
+    # the band edges: T/2 - 1 rows predicted heavy fails, T/2 passes; 2T - 1 missed passes, 2T fails
+    T = n_p // 1024
+    for count, pred, ok in [(T // 2 - 1, True, False), (T // 2, True, True), (2 * T - 1, False, True),
+                            (2 * T, False, False)]:
+        ex2 = ex.copy()
+        ex2[5] = count
+        heavy_pred = (0, 1, 2, 5) if pred else (0, 1, 2)
+        plan, fb = planner.decode(_logits(2, 2, 1, heavy=heavy_pred), cc, ex2, n_b, n_p, 4)
+        assert fb == (not ok), (count, pred)
+
+
+def test_decode_accepts_either_answer_near_the_threshold(pj):
+    from partitioned_join_amd import planner
+    n_b, n_p = 1 << 26, 1 << 28
+    cc, ex = _cands(12, heavy=(0,), border=(1, 2), n_probe=n_p)
+    for heavy_pred in [(0,), (0, 1), (0, 1, 2)]:
+        plan, fb = planner.decode(_logits(2, 2, 1, heavy=heavy_pred), cc, ex, n_b, n_p, 4)
+        assert not fb
+
+
+def test_heavy_targets_follow_the_band(pj):
+    from partitioned_join_amd import planner
+    n = 1 << 20
+    T = n // 1024
+    t = planner.heavy_targets(np.array([3, 3, 3, 3, 0]), np.array([2 * T, 2 * T - 1, T // 2, T // 2 - 1, 5 * T]), n)
+    assert t.tolist() == [1.0, -1.0, -1.0, 0.0, -1.0]
 
 
 def test_decode_rejects_plans_whose_chunks_do_not_fit(pj):
