@@ -15,6 +15,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -86,6 +88,37 @@ def measured_traffic(kernel):
     return None if rec is None else rec.get("bytes_per_launch")
 
 
+def batched_joins(pj, torch, count=512):
+    """512 independent int64-key joins of uneven sizes replayed as one HIP
+    graph: the untuned default (8 chains, plain plans) and the autotuned
+    configuration over chains x fan-out with learned per-join plans."""
+    from partitioned_join_amd import batch
+    builds, probes, specs = batch.make_batch_workload(count)
+    tuples = sum(s["n_build"] + s["n_probe"] for s in specs)
+    plain = batch.batch_plans(builds, probes, "plain")
+    bj = batch.BatchJoin(builds, probes, plain, chains=8)
+    ms_default = bj.time(3)
+    counts_default = bj.counts()
+    bj.close()
+    learned = batch.batch_plans(builds, probes, "learned")
+    t0 = time.perf_counter()
+    best, cfg, timings = batch.autotune(builds, probes, learned, chain_options=(8, 16, 32), bit_offsets=(-1, 0, 1),
+                                        reps=3)
+    tune_s = time.perf_counter() - t0
+    ms_best = best.time(5)
+    counts_best = best.counts()
+    best.close()
+    assert np.array_equal(counts_default, counts_best)
+    return {"workload": f"{count} joins, |R| log-uniform [10K,4M], |S| 1-8x|R|, uniform 64-bit int64 build keys, "
+                        "Zipf(theta~U[0,1.2]) probes, int32 row-id payloads, one HIP graph",
+            "tuples": tuples, "pairs": int(counts_best.sum()),
+            "default_ms": ms_default, "default_tuples_per_s": tuples / (ms_default / 1000.0),
+            "tuned_ms": ms_best, "tuned_tuples_per_s": tuples / (ms_best / 1000.0),
+            "tuned_config": {"chains": cfg[0], "fanout_offset_bits": cfg[1], "plans": "learned with plain fallback"},
+            "autotune_seconds": tune_s,
+            "timings_ms": {f"chains={c},fanout{d:+d}": v for (c, d), v in timings.items()}}
+
+
 def cpu_baseline(scale_shift=2):
     """Reference inner_join_cpu (oracle/_ref) on the same distribution at
     1/2^scale_shift size; falls back to the oracle restatement if the
@@ -119,6 +152,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-skew", action="store_true", help="skip the Zipf(1.0) comparison")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-joins graph measurement")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a 1 GiB device copy (counter calibration for rocprofv3 --pmc runs)")
     args = ap.parse_args()
@@ -192,6 +226,8 @@ def main():
         line["zipf_theta1"] = rec
         del zb, zp
         torch.cuda.empty_cache()
+    if not args.no_batch:
+        line["batched"] = batched_joins(pj, torch)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline()
     print(json.dumps(line), flush=True)

@@ -323,6 +323,28 @@ size_t pj_plan_learned_workspace_size(void);
 pj_status pj_plan_learned(const pj_table* build, const pj_table* probe, const float* d_weights, pj_plan* plan,
                           int32_t* fallback, void* workspace, size_t workspace_bytes, hipStream_t stream);
 
+/* ------------------------------------------------------------------------ */
+/* Batched joins.  No reference counterpart (the reference runs one join per  */
+/* process).  Many independent joins are captured once into one HIP graph:    */
+/* the joins are spread over `chains` parallel chains (longest first by      */
+/* |R| + |S|), each chain runs its joins' pipelines one after another on its */
+/* own region of the workspace, and one pj_batch_launch replays the whole    */
+/* graph.  Join j writes at most out_caps[j] pairs at out + out_offsets[j]    */
+/* and its pair count to d_counts[j] (device), as pj_partitioned_join_async  */
+/* does.  The tables, output and workspace must stay allocated and unchanged */
+/* in address while the batch exists; their contents may change between      */
+/* launches.                                                                 */
+/* ------------------------------------------------------------------------ */
+typedef struct pj_batch pj_batch;
+/* plans: NULL (plain plan for every join) or one plan per join. */
+pj_status pj_batch_workspace_size(const pj_table* builds, const pj_table* probes, const pj_plan* plans,
+                                  int32_t count, int32_t chains, size_t* bytes);
+pj_status pj_batch_create(const pj_table* builds, const pj_table* probes, const pj_plan* plans, int32_t count,
+                          int32_t chains, pj_pair* out, const int64_t* out_offsets, const int64_t* out_caps,
+                          int64_t* d_counts, void* workspace, size_t workspace_bytes, pj_batch** batch);
+pj_status pj_batch_launch(pj_batch* batch, hipStream_t stream);
+pj_status pj_batch_destroy(pj_batch* batch);
+
 #ifdef __cplusplus
 }
 #endif

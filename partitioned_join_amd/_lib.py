@@ -98,6 +98,11 @@ SIGNATURES = {
     "pj_plan_decode": (c_i32, [c_vp, c_vp, c_vp, P_TABLE, P_TABLE, P_PLAN, ctypes.POINTER(c_i32)]),
     "pj_plan_learned_workspace_size": (c_size, []),
     "pj_plan_learned": (c_i32, [P_TABLE, P_TABLE, c_vp, P_PLAN, ctypes.POINTER(c_i32), c_vp, c_size, c_vp]),
+    "pj_batch_workspace_size": (c_i32, [P_TABLE, P_TABLE, P_PLAN, c_i32, c_i32, ctypes.POINTER(c_size)]),
+    "pj_batch_create": (c_i32, [P_TABLE, P_TABLE, P_PLAN, c_i32, c_i32, c_vp, c_vp, c_vp, c_vp, c_vp, c_size,
+                                ctypes.POINTER(c_vp)]),
+    "pj_batch_launch": (c_i32, [c_vp, c_vp]),
+    "pj_batch_destroy": (c_i32, [c_vp]),
 }
 
 _LIB = None

@@ -5,11 +5,14 @@
 // runs with no host round trip, on caller-provided workspace.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <numeric>
 #include <string>
+#include <vector>
 
 #include "../../include/pj.h"
 #include "pj_device.h"
@@ -875,6 +878,195 @@ pj_status pj_plan_learned(const pj_table* build, const pj_table* probe, const fl
   PJ_HIP(hipMemcpyAsync(h_exact, exact, sizeof(h_exact), hipMemcpyDeviceToHost, stream));
   PJ_HIP(hipStreamSynchronize(stream));
   return pj_plan_decode(h_logits, h_cand_n, h_exact, build, probe, plan, fallback);
+}
+
+// ---------------------------------------------------------------------------
+// Batched joins: one graph of many independent join pipelines
+// ---------------------------------------------------------------------------
+}  // extern "C"
+
+struct pj_batch {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int32_t count = 0;
+  int32_t chains = 0;
+};
+
+namespace pj {
+
+// Longest-first assignment of the joins to `chains` chains by |R| + |S| (plus
+// a fixed cost per join for its launches), so the chains finish together.
+static std::vector<int> assign_chains(const pj_table* builds, const pj_table* probes, int count, int chains) {
+  std::vector<int> order(count), chain_of(count);
+  std::iota(order.begin(), order.end(), 0);
+  auto cost = [&](int j) { return double(builds[j].n) + double(probes[j].n) + 2.0e5; };
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost(a) > cost(b); });
+  std::vector<double> load(chains, 0.0);
+  for (int j : order) {
+    const int c = int(std::min_element(load.begin(), load.end()) - load.begin());
+    chain_of[j] = c;
+    load[c] += cost(j);
+  }
+  return chain_of;
+}
+
+// Workspace of each chain (the largest of its joins), 256-byte aligned.
+static pj_status chain_workspaces(const pj_table* builds, const pj_table* probes, const pj_plan* plans, int count,
+                                  int chains, const std::vector<int>& chain_of, std::vector<size_t>* off,
+                                  size_t* total) {
+  std::vector<size_t> need(chains, 0);
+  for (int j = 0; j < count; ++j) {
+    size_t b = 0;
+    pj_status st = pj_workspace_size(&builds[j], &probes[j], plans ? &plans[j] : nullptr, &b);
+    if (st != PJ_OK) return st;
+    need[chain_of[j]] = std::max(need[chain_of[j]], (b + 255) & ~size_t(255));
+  }
+  off->assign(chains, 0);
+  size_t acc = 0;
+  for (int c = 0; c < chains; ++c) {
+    (*off)[c] = acc;
+    acc += need[c];
+  }
+  *total = acc;
+  return PJ_OK;
+}
+
+static pj_status check_batch(const pj_table* builds, const pj_table* probes, int32_t count, int32_t chains) {
+  if (!builds || !probes || count < 1) return fail(PJ_ERR_INVALID_ARG, "a batch needs count >= 1 tables");
+  if (chains < 1 || chains > 64) return fail(PJ_ERR_INVALID_ARG, "chains must be in [1, 64] (got %d)", chains);
+  for (int j = 0; j < count; ++j) {
+    pj_status st = check_pair(&builds[j], &probes[j], false);
+    if (st != PJ_OK) return fail(st, "join %d: %s", j, g_err.c_str());
+  }
+  return PJ_OK;
+}
+
+}  // namespace pj
+
+extern "C" {
+
+pj_status pj_batch_workspace_size(const pj_table* builds, const pj_table* probes, const pj_plan* plans,
+                                  int32_t count, int32_t chains, size_t* bytes) {
+  pj_status st = check_batch(builds, probes, count, chains);
+  if (st != PJ_OK) return st;
+  if (!bytes) return fail(PJ_ERR_INVALID_ARG, "bytes is NULL");
+  const std::vector<int> chain_of = assign_chains(builds, probes, count, chains);
+  std::vector<size_t> off;
+  return chain_workspaces(builds, probes, plans, count, chains, chain_of, &off, bytes);
+}
+
+pj_status pj_batch_create(const pj_table* builds, const pj_table* probes, const pj_plan* plans, int32_t count,
+                          int32_t chains, pj_pair* out, const int64_t* out_offsets, const int64_t* out_caps,
+                          int64_t* d_counts, void* workspace, size_t workspace_bytes, pj_batch** batch) {
+  pj_status st = check_batch(builds, probes, count, chains);
+  if (st != PJ_OK) return st;
+  if (!out_offsets || !out_caps || !d_counts || !batch || !workspace)
+    return fail(PJ_ERR_INVALID_ARG, "NULL batch argument");
+  *batch = nullptr;
+  for (int j = 0; j < count; ++j) {
+    st = check_pair(&builds[j], &probes[j]);
+    if (st != PJ_OK) return fail(st, "join %d: %s", j, g_err.c_str());
+    if (out_offsets[j] < 0 || out_caps[j] < 0 || (out_caps[j] > 0 && !out))
+      return fail(PJ_ERR_INVALID_ARG, "join %d: bad output offset or capacity", j);
+  }
+  const std::vector<int> chain_of = assign_chains(builds, probes, count, chains);
+  std::vector<size_t> off;
+  size_t need = 0;
+  st = chain_workspaces(builds, probes, plans, count, chains, chain_of, &off, &need);
+  if (st != PJ_OK) return st;
+  if (workspace_bytes < need)
+    return fail(PJ_ERR_WORKSPACE, "batch workspace of %zu bytes is smaller than the %zu required", workspace_bytes,
+                need);
+  if (reinterpret_cast<uintptr_t>(workspace) % 256 != 0)
+    return fail(PJ_ERR_INVALID_ARG, "workspace must be 256-byte aligned");
+
+  // Capture: a fork to one stream per chain, each join's pipeline in its
+  // chain, a join back to the capture stream.  Profiling events stay off.
+  const bool prof = g_prof.on;
+  g_prof.on = false;
+  hipStream_t cs = nullptr;
+  std::vector<hipStream_t> streams(chains, nullptr);
+  std::vector<hipEvent_t> done(chains, nullptr);
+  hipEvent_t fork = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  pj_status result = PJ_OK;
+  bool capturing = false;
+  auto hip_ok = [&](hipError_t e, const char* what) {
+    if (e == hipSuccess) return true;
+    result = fail(PJ_ERR_HIP, "%s failed: %s", what, hipGetErrorString(e));
+    return false;
+  };
+  do {
+    if (!hip_ok(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking), "hipStreamCreateWithFlags")) break;
+    bool ok = true;
+    for (int c = 0; c < chains && ok; ++c)
+      ok = hip_ok(hipStreamCreateWithFlags(&streams[c], hipStreamNonBlocking), "hipStreamCreateWithFlags") &&
+           hip_ok(hipEventCreateWithFlags(&done[c], hipEventDisableTiming), "hipEventCreateWithFlags");
+    if (!ok || !hip_ok(hipEventCreateWithFlags(&fork, hipEventDisableTiming), "hipEventCreateWithFlags")) break;
+    if (!hip_ok(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture")) break;
+    capturing = true;
+    if (!hip_ok(hipEventRecord(fork, cs), "hipEventRecord")) break;
+    for (int c = 0; c < chains && ok; ++c) ok = hip_ok(hipStreamWaitEvent(streams[c], fork, 0), "hipStreamWaitEvent");
+    if (!ok) break;
+    for (int j = 0; j < count && ok; ++j) {
+      const int c = chain_of[j];
+      char* ws = static_cast<char*>(workspace) + off[c];
+      const size_t ws_bytes = (c + 1 < chains ? off[c + 1] : need) - off[c];
+      const pj_status js = run_join(&builds[j], &probes[j], plans ? &plans[j] : nullptr, out + out_offsets[j],
+                                    out_caps[j], d_counts + j, ws, ws_bytes, streams[c]);
+      if (js != PJ_OK) {
+        result = fail(js, "join %d: %s", j, g_err.c_str());
+        ok = false;
+      }
+    }
+    if (!ok) break;
+    for (int c = 0; c < chains && ok; ++c)
+      ok = hip_ok(hipEventRecord(done[c], streams[c]), "hipEventRecord") &&
+           hip_ok(hipStreamWaitEvent(cs, done[c], 0), "hipStreamWaitEvent");
+    if (!ok) break;
+    capturing = false;
+    if (!hip_ok(hipStreamEndCapture(cs, &graph), "hipStreamEndCapture")) break;
+    if (!hip_ok(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "hipGraphInstantiate")) break;
+  } while (false);
+  if (capturing) {
+    hipGraph_t g = nullptr;
+    (void)hipStreamEndCapture(cs, &g);
+    if (g) (void)hipGraphDestroy(g);
+  }
+  g_prof.on = prof;
+  for (int c = 0; c < chains; ++c) {
+    if (done[c]) (void)hipEventDestroy(done[c]);
+    if (streams[c]) (void)hipStreamDestroy(streams[c]);
+  }
+  if (fork) (void)hipEventDestroy(fork);
+  if (cs) (void)hipStreamDestroy(cs);
+  if (result != PJ_OK) {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    return result;
+  }
+  pj_batch* b = new pj_batch;
+  b->graph = graph;
+  b->exec = exec;
+  b->count = count;
+  b->chains = chains;
+  *batch = b;
+  return PJ_OK;
+}
+
+pj_status pj_batch_launch(pj_batch* batch, hipStream_t stream) {
+  if (!batch || !batch->exec) return fail(PJ_ERR_INVALID_ARG, "batch is NULL");
+  PJ_HIP(hipGraphLaunch(batch->exec, stream));
+  return PJ_OK;
+}
+
+pj_status pj_batch_destroy(pj_batch* batch) {
+  if (!batch) return PJ_OK;
+  if (batch->exec) (void)hipGraphExecDestroy(batch->exec);
+  if (batch->graph) (void)hipGraphDestroy(batch->graph);
+  delete batch;
+  return PJ_OK;
 }
 
 }  // extern "C"

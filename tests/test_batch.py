@@ -1,0 +1,131 @@
+"""Batched joins as one HIP graph: workspace sizing and argument checks (CPU),
+and on the GPU every replay of the graph -- with eager work and host
+synchronisations between replays, and with input contents changed in place --
+against the same joins run eagerly and against the oracle."""
+import ctypes
+
+import numpy as np
+import pytest
+
+
+def _size_tables(sizes, kb=8):
+    from partitioned_join_amd._lib import PjTable
+    n = len(sizes)
+    tb = (PjTable * n)(*[PjTable(None, None, b, kb, 0) for b, _ in sizes])
+    tp = (PjTable * n)(*[PjTable(None, None, p, kb, 0) for _, p in sizes])
+    return tb, tp
+
+
+def test_batch_workspace_and_errors(pj):
+    L = pj.lib()
+    sizes = [(10000, 30000), (200000, 900000), (5000, 5000), (1 << 20, 1 << 22)]
+    tb, tp = _size_tables(sizes)
+    ws = []
+    for chains in (1, 2, 4):
+        b = ctypes.c_size_t(0)
+        assert L.pj_batch_workspace_size(tb, tp, None, len(sizes), chains, ctypes.byref(b)) == 0
+        ws.append(b.value)
+    # one chain holds the largest join's workspace; more chains hold more regions
+    one = ctypes.c_size_t(0)
+    t1b, t1p = _size_tables([sizes[3]])
+    L.pj_batch_workspace_size(t1b, t1p, None, 1, 1, ctypes.byref(one))
+    assert ws[0] == one.value and ws[0] < ws[1] <= ws[2]
+    b = ctypes.c_size_t(0)
+    assert L.pj_batch_workspace_size(tb, tp, None, 4, 0, ctypes.byref(b)) == 1
+    assert L.pj_batch_workspace_size(tb, tp, None, 0, 2, ctypes.byref(b)) == 1
+    assert L.pj_batch_launch(None, None) == 1
+    assert L.pj_batch_destroy(None) == 0
+
+
+def test_shift_plan_bounds(pj):
+    from partitioned_join_amd import batch
+    p = pj.plan_default(1 << 20, 1 << 22, 8)
+    up = batch.shift_plan(p, 1, 1 << 20, 8)
+    down = batch.shift_plan(p, -1, 1 << 20, 8)
+    assert up.bits1 + up.bits2 == p.bits1 + p.bits2 + 1
+    # one bit fewer would make the average build chunk 1.5 * 2^20 / 2^(B-1) exceed the LDS table
+    assert down.bits1 + down.bits2 in (p.bits1 + p.bits2 - 1, p.bits1 + p.bits2)
+    big = pj.plan_default(1 << 26, 1 << 28, 4)
+    assert batch.shift_plan(big, 1, 1 << 26, 4).bits1 + batch.shift_plan(big, 1, 1 << 26, 4).bits2 == 16
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch as t
+    assert t.cuda.is_available(), "GPU tests need a HIP device"
+    return t
+
+
+def _small_batch(torch, pj, count=24, seed=9):
+    G = pj.generators
+    rng = np.random.default_rng(seed)
+    builds, probes = [], []
+    for j in range(count):
+        nb = int(rng.integers(1000, 120000))
+        ns = int(nb * rng.uniform(1, 6))
+        bk = G.build_i64(nb, 100 + j)
+        if j % 3 == 0:
+            pk = G.probe_i64(ns, nb, 100 + j, 500 + j, 0.7)
+        else:
+            pk = G.probe_zipf_i64(ns, nb, float(rng.uniform(0, 1.2)), 100 + j, 600 + j, 700 + j)
+        builds.append(bk)
+        probes.append(pk)
+    builds.append(torch.empty(0, dtype=torch.int64, device="cuda"))  # an empty join in the batch
+    probes.append(G.probe_i64(100, 10, 1, 2, 0.5))
+    return builds, probes
+
+
+def _eager(pj, oracle, builds, probes, plans=None):
+    out = []
+    for j, (b, p) in enumerate(zip(builds, probes)):
+        pairs = pj.partitioned_join(b, p, plan=None if plans is None else plans[j])
+        out.append(oracle.sort_pairs(pairs.cpu().numpy()))
+    return out
+
+
+@pytest.mark.gpu
+def test_batch_graph_replays_match_eager(pj, oracle, torch):
+    from partitioned_join_amd import batch
+    builds, probes = _small_batch(torch, pj)
+    expect = _eager(pj, oracle, builds, probes)
+    for j in (1, 5):
+        ref = oracle.sort_pairs(oracle.inner_join_cpu(builds[j].cpu().numpy(), probes[j].cpu().numpy()))
+        assert np.array_equal(expect[j], ref)
+    bj = batch.BatchJoin(builds, probes, chains=4)
+    for replay in range(3):
+        bj.launch()
+        torch.cuda.synchronize()
+        counts = bj.counts()
+        for j in range(len(builds)):
+            assert counts[j] == len(expect[j]), (replay, j)
+            assert np.array_equal(oracle.sort_pairs(bj.pairs(j, counts).cpu().numpy()), expect[j]), (replay, j)
+        # eager work between replays, on the same stream and device memory
+        x = torch.randn(1 << 20, device="cuda")
+        assert float(torch.sort(x)[0][-1]) >= float(x.max()) - 1e-6
+        bj.out.fill_(-7)
+        bj.d_counts.fill_(-1)
+    # the graph reads the inputs' current contents: regenerate one probe table in place
+    G = pj.generators
+    new_p = G.probe_i64(probes[2].numel(), builds[2].numel(), 102, 999, 0.3)
+    probes[2].copy_(new_p)
+    bj.launch()
+    torch.cuda.synchronize()
+    exp2 = _eager(pj, oracle, [builds[2]], [probes[2]])[0]
+    assert np.array_equal(oracle.sort_pairs(bj.pairs(2).cpu().numpy()), exp2)
+    bj.close()
+
+
+@pytest.mark.gpu
+def test_batch_autotune_and_learned_plans(pj, oracle, torch):
+    from partitioned_join_amd import batch
+    builds, probes = _small_batch(torch, pj, count=12, seed=10)
+    expect = _eager(pj, oracle, builds, probes)
+    plans = batch.batch_plans(builds, probes, "learned")
+    bj, cfg, timings = batch.autotune(builds, probes, plans, chain_options=(2, 4), bit_offsets=(-1, 0, 1), reps=2)
+    assert len(timings) == 6 and cfg in timings
+    bj.launch()
+    torch.cuda.synchronize()
+    counts = bj.counts()
+    for j in range(len(builds)):
+        assert np.array_equal(oracle.sort_pairs(bj.pairs(j, counts).cpu().numpy()), expect[j]), j
+    bj.close()
