@@ -757,7 +757,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
   const int F1 = pp.F1, F2 = pp.F2, b2 = pp.b2, P = pp.P;
   const uint32_t mask = uint32_t(F2 - 1);
   const uint32_t nseg = T.nseg;
-  const uint32_t NG = nseg < uint32_t(kPass2Groups) ? nseg : uint32_t(kPass2Groups);
+  const uint32_t NG = T.ng;  // 1 <= NG <= nseg
   const int n_p1 = F1 > x ? (F1 - x + kNumXcd - 1) / kNumXcd : 0;
   const uint32_t items = uint32_t(n_p1) * NG;
 
@@ -870,10 +870,23 @@ hipError_t launch_pass2(const Layout& L, int key_bytes, hipStream_t s) {
   }
   const uint32_t g0 = x0 * uint32_t(kNumXcd);
   dim3 grid(per_xcd * kNumXcd), block(kScatterBlock);
+  // Items of about kPass2ItemTiles full tiles: a pass-1 partition of a small
+  // table is split into fewer segment groups, so per-item costs (offsets,
+  // an unprefetched first tile, the partial tail) stay small next to the work.
+  PartTable t[2] = {L.t[0], L.t[1]};
+  const uint32_t tile = uint32_t(key_bytes == 4 ? kScatterTile4 : kScatterTile8) / (L.pp.F2 <= 1024 ? 1u : 2u);
+  for (int i = 0; i < L.ntables; ++i) {
+    const uint64_t per_p1 = uint64_t(t[i].n) / uint64_t(L.pp.F1);
+    uint64_t ng = per_p1 / (uint64_t(kPass2ItemTiles) * tile);
+    if (ng > uint64_t(kPass2Groups)) ng = kPass2Groups;
+    if (ng > t[i].nseg) ng = t[i].nseg;
+    if (ng < 1) ng = 1;
+    t[i].ng = uint32_t(ng);
+  }
   const bool big = L.pp.F2 <= 1024;
 #define PJ_PASS2(KB_, TILE_)                                                                                     \
   hipLaunchKernelGGL((k_pass2<KB_, kScatterBlock, TILE_>), grid, block, (scatter_lds_bytes<KB_, TILE_>(L.pp.F2)), s, \
-                     L.t[0], L.t[1], L.pp, g0)
+                     t[0], t[1], L.pp, g0)
   if (key_bytes == 4) {
     if (big) PJ_PASS2(4, kScatterTile4);
     else PJ_PASS2(4, kScatterTile4 / 2);
