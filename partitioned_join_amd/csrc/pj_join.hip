@@ -300,9 +300,6 @@ __global__ __launch_bounds__(BLOCK) void k_join(JoinArgs A) {
     }
   };
   if (d.valid) load_item(d, bt, pt);
-  const uint32_t dmask = uint32_t(A.P) - 1u;  // partition digit of a key: its low bits
-  uint32_t prev_q = 0, fill_digit = 0;
-  bool fresh = true;  // the table holds nothing of this launch yet
 
   while (d.valid) {
     tup_t bn[NB], pn[R];
@@ -310,20 +307,11 @@ __global__ __launch_bounds__(BLOCK) void k_join(JoinArgs A) {
     const uint64_t item_nn = item + 2 * uint64_t(gridDim.x);
     raw_desc_load(A, item_nn, ra, rb);  // decoded at the end of this item
     const uint32_t tsize = uint32_t(d.fd.d) * BS;
-    // A slot is free when its key's partition digit is not q: every key of
-    // this item has digit q, so the keys of the workgroup's earlier items
-    // (smaller digits: items come in partition order) are free as they lie.
-    // The whole table is refilled only where a slot could hold digit q: for
-    // the first item, after an item of the same partition, and when q is the
-    // digit of the last fill.  A fill uses digit q - 1, which no later item
-    // has (P - 1 for q = 0).
-    const uint32_t q = d.q;
-    auto is_free = [&](key_t k) { return (uint32_t(k) & dmask) != q; };
-    if (fresh || q == prev_q || q == fill_digit) {
-      fill_digit = (q - 1u) & dmask;
-      const uint4 fill = bucket_fill(key_t(fill_digit));
+    const key_t sentinel = key_t(d.q ^ 1u);
+    {
+      const uint4 fill = bucket_fill(sentinel);
       uint4* s4 = reinterpret_cast<uint4*>(s_keys);
-      for (uint32_t i = tid; i < uint32_t(A.table_cap) / BS; i += BLOCK) s4[i] = fill;
+      for (uint32_t i = tid; i < tsize / BS; i += BLOCK) s4[i] = fill;
     }
     lds_barrier();
     // The previous item's matches go out now, after this item's prefetch was
@@ -345,23 +333,20 @@ __global__ __launch_bounds__(BLOCK) void k_join(JoinArgs A) {
         bslot[k] = v ? BS * hash_slot(murmur3(KTr::key(bt[k])), d.fd) : 0u;
       }
       while (pend) {
-        key_t cur[NB], old[NB];
+        key_t old[NB];
 #pragma unroll
         for (int k = 0; k < NB; ++k)
-          if (pend & (1u << k)) cur[k] = s_keys[bslot[k]];
-#pragma unroll
-        for (int k = 0; k < NB; ++k)
-          if (pend & (1u << k)) old[k] = is_free(cur[k]) ? lds_cas(&s_keys[bslot[k]], cur[k], KTr::key(bt[k])) : cur[k];
+          if (pend & (1u << k)) old[k] = lds_cas(&s_keys[bslot[k]], sentinel, KTr::key(bt[k]));
 #pragma unroll
         for (int k = 0; k < NB; ++k)
           if (pend & (1u << k)) {
-            if (is_free(cur[k]) && old[k] == cur[k]) {  // claimed
+            if (old[k] == sentinel) {
               s_pays[bslot[k]] = KTr::pay(bt[k]);
               pend &= ~(1u << k);
-            } else if (!is_free(old[k])) {  // taken by a key of this item
+            } else {
               dup |= old[k] == KTr::key(bt[k]);
               bslot[k] = bslot[k] + 1 == tsize ? 0u : bslot[k] + 1;
-            }  // else the slot is still free: try it again
+            }
           }
       }
     }
@@ -395,7 +380,7 @@ __global__ __launch_bounds__(BLOCK) void k_join(JoinArgs A) {
 #pragma unroll
             for (int i = BS - 1; i >= 0; --i) {
               hit = kk[i] == key ? uint32_t(i) : hit;
-              empty |= is_free(kk[i]) ? 1u : 0u;
+              empty |= kk[i] == sentinel ? 1u : 0u;
             }
             if (hit < BS) {
               found |= 1u << k;
@@ -453,7 +438,7 @@ __global__ __launch_bounds__(BLOCK) void k_join(JoinArgs A) {
           for (int k = 0; k < R; ++k)
             if (walk & (1u << k)) {
               if (e[k] == KTr::key(ptr[k])) hit |= 1u << k;
-              else if (is_free(e[k])) walk &= ~(1u << k);
+              else if (e[k] == sentinel) walk &= ~(1u << k);
               else slot[k] = slot[k] + 1 == tsize ? 0u : slot[k] + 1;
             }
           if (__ballot(hit != 0)) {
@@ -510,8 +495,6 @@ __global__ __launch_bounds__(BLOCK) void k_join(JoinArgs A) {
     // the next item may overwrite the table, and the stage is copied out after
     // the next item's first barrier.
     if (tid == 0) plan_copy<STAGE>(sink, &ctl, ctl.n);
-    prev_q = q;
-    fresh = false;
     // advance the pipeline
     d = dn;
     dn = desc_from_raw(A, ra, rb, item_nn, total_items);
