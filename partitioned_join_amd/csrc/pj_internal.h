@@ -21,9 +21,9 @@ constexpr int kJoinLdsBytes = 81408;  // dynamic LDS per join workgroup: two wor
 constexpr int64_t kOutChunk = 16384;  // output pairs claimed per global allocation
 constexpr int kJoinGrid = 512;        // persistent join workgroups (2 per CU)
 constexpr int kPass2WgPerXcd = 16;    // pass-2 persistent workgroups per XCD and table (one per CU)
-constexpr int kPass2Groups = 16;      // at most this many segment groups per pass-1 partition in pass 2
 constexpr int kPass2ItemTiles = 4;    // pass-2 items hold about this many full tiles
 constexpr int kMaxTotalBits = 16;     // combined histogram limit (65536 partitions)
+constexpr int kMaxRadix = 4096;       // largest fan-out of one pass (12 bits)
 constexpr int kMaxSegPerXcd = 64;
 
 // One table being partitioned.  All pointers are device pointers.
@@ -39,7 +39,6 @@ struct PartTable {
   uint32_t* tot;         // [P] fine totals
   uint32_t* totc;        // [F1] pass-1 totals (2-pass)
   uint32_t* off;         // [P+1] final partition starts, off[P] = n
-  uint32_t ng;           // pass-2 segment groups per pass-1 partition (set by the pass-2 launcher)
 };
 
 struct PartParams {
