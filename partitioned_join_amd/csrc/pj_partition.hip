@@ -448,16 +448,89 @@ __device__ __forceinline__ void rank_tile(const uint32_t (&dg)[E], uint32_t* s_c
     pk[e] = dg[e] != 0xFFFFFFFFu ? (dg[e] << 16) | atomicAdd(&s_cnt[dg[e]], 1u) : 0xFFFFFFFFu;
 }
 
+// The scatter kernels hold and move tuples as vectors of the tuple's bytes
+// (uint2: key, payload; uint4: key low, key high, payload, 0): a struct value
+// merged over branches or copied between register sets is kept in scratch
+// memory, whose accesses join the in-order memory counter.  The key's low
+// 32 bits are word x in both.
+template <int KB>
+struct ScatterTup {
+  using type = std::conditional_t<KB == 4, uint2, uint4>;
+  static_assert(sizeof(type) == sizeof(typename KeyT<KB>::tup_t), "tuple and vector sizes differ");
+  __device__ __forceinline__ static type make(typename KeyT<KB>::key_t k, int32_t p) {
+    if constexpr (KB == 4) return make_uint2(uint32_t(k), uint32_t(p));
+    else return make_uint4(uint32_t(k), uint32_t(uint64_t(k) >> 32), uint32_t(p), 0u);
+  }
+};
+
+// Write combining.  HBM3E writes whole 128-byte lines: a store that covers
+// part of a line makes the memory controller read the rest first, and a
+// scatter whose runs start and end mid-line moves at ~3.7 TB/s where whole
+// lines move at ~5 TB/s on MI355X.  So each digit keeps the unfinished last
+// line of its run in LDS (line[d * LT + position % LT]) and a tile's flush
+// stores only lines that are complete; a carried line goes out when a later
+// tile completes it, or with the range's last tile.  A lane with nothing to
+// store writes its sink slot instead (a private, never-read line of the
+// workgroup), so every thread issues a fixed number of stores and the
+// compiler's waits for the next tile's loads stay exact.  CF is the largest
+// digit count of the instantiation (0: no carry).
+template <int KB>
+struct Carry {
+  using vec_t = typename ScatterTup<KB>::type;
+  static constexpr int LT = 128 / int(sizeof(vec_t));  // tuples per 128-byte line
+  vec_t* line;            // [CF][LT] line images
+  const uint32_t* start;  // [F] first position of the range per digit
+  vec_t* out;             // the pass's output table
+  vec_t* sink;            // this lane's sink slot
+};
+
+// Per workgroup slot and lane, 16 bytes each; only ever written.
+constexpr int kSinkSlots = 1024;
+static __device__ uint4 g_scatter_sink[kSinkSlots * kWave];
+
+template <int KB>
+__device__ __forceinline__ typename ScatterTup<KB>::type* sink_slot() {
+  const uint32_t wg = (blockIdx.y * gridDim.x + blockIdx.x) % uint32_t(kSinkSlots);
+  return reinterpret_cast<typename ScatterTup<KB>::type*>(g_scatter_sink + size_t(wg) * kWave + lane_id());
+}
+
+// After a tile's scan (s_cur advanced by the tile's counts s_cnt): the carried
+// line of each digit whose run the tile completes (or every one, for the last
+// tile) goes out -- its positions from the range start or the line start,
+// whichever is later, up to the old cursor.  ceil(CF * LT / BLOCK) stores per
+// thread.
+template <int KB, int BLOCK, int CF>
+__device__ __forceinline__ void carry_out(int F, const uint32_t* s_cnt, const uint32_t* s_cur, const Carry<KB>& cy,
+                                          bool last) {
+  constexpr uint32_t LT = uint32_t(Carry<KB>::LT), NS = uint32_t(CF) * LT;
+  // The thread index through an opaque move: the slot addresses below are
+  // then recomputed per tile (a few adds) instead of being hoisted out of
+  // the tile loop, where they would be spilled, and a spill reload waits for
+  // every outstanding load and store.
+  uint32_t tid;
+  __asm__ volatile("v_mov_b32 %0, %1" : "=v"(tid) : "v"(uint32_t(threadIdx.x)));
+#pragma unroll
+  for (int r = 0; r < int((NS + BLOCK - 1) / BLOCK); ++r) {
+    const uint32_t slot = min(uint32_t(r) * BLOCK + tid, NS - 1), d = slot / LT, k = slot % LT;
+    const uint32_t dd = d < uint32_t(F) ? d : uint32_t(F) - 1;
+    const uint32_t cur = s_cur[dd], old = cur - s_cnt[dd], l0 = old & ~(LT - 1), pos = l0 + k;
+    const bool done = last || (cur & ~(LT - 1)) > l0;
+    const bool valid = uint32_t(r) * BLOCK + tid < NS && d < uint32_t(F) && done && pos < old && pos >= cy.start[dd];
+    const typename Carry<KB>::vec_t v = cy.line[slot];
+    *(valid ? cy.out + pos : cy.sink) = v;
+  }
+}
+
 // After rank_tile: the first wave scans the F digit counts (F / 64 per lane,
 // no barrier), advances the per-digit global cursors s_cur and records s_delta
 // (global minus staged position per digit); then the tile is staged in digit
 // order while s_cnt_next is cleared for the next tile.  Three barriers; ends
 // with one.
-template <int KB, int BLOCK, int E>
-__device__ __forceinline__ void scan_stage_tile(const typename KeyT<KB>::tup_t (&tup)[E], const uint32_t (&pk)[E],
-                                                int F, typename KeyT<KB>::tup_t* s_stage, const uint32_t* s_cnt,
+template <int KB, int BLOCK, int E, int CF>
+__device__ __forceinline__ void scan_stage_tile(const typename ScatterTup<KB>::type (&tup)[E], const uint32_t (&pk)[E],
+                                                int F, typename ScatterTup<KB>::type* s_stage, const uint32_t* s_cnt,
                                                 uint32_t* s_cnt_next, uint32_t* s_base, uint32_t* s_cur,
-                                                uint32_t* s_delta) {
+                                                uint32_t* s_delta, const Carry<KB>& cy, bool last) {
   const int tid = threadIdx.x;
   lds_barrier();
   if (tid < kWave) {
@@ -485,6 +558,7 @@ __device__ __forceinline__ void scan_stage_tile(const typename KeyT<KB>::tup_t (
     }
   }
   lds_barrier();
+  if constexpr (CF > 0) carry_out<KB, BLOCK, CF>(F, s_cnt, s_cur, cy, last);
   for (int d = tid; d < F; d += BLOCK) s_cnt_next[d] = 0;
 #pragma unroll
   for (int e = 0; e < E; ++e)
@@ -496,45 +570,80 @@ __device__ __forceinline__ void scan_stage_tile(const typename KeyT<KB>::tup_t (
 // so the compiler knows how many memory operations follow the next tile's
 // loads and waits for those alone.  A destination past the table (impossible
 // with exact offsets) is clamped to its last row rather than faulting.
-template <int KB, int BLOCK, int E>
-__device__ __forceinline__ void flush_full(int shift, uint32_t mask, const typename KeyT<KB>::tup_t* s_stage,
-                                           const uint32_t* s_delta, typename KeyT<KB>::tup_t* out, uint32_t n_out) {
-  using KTr = KeyT<KB>;
+// With a carry (CF > 0), tuples in the unfinished last line of their digit's
+// run go to the line image instead (and the lane stores to its sink); after
+// the range's last tile (last) nothing is held back.
+template <int KB, int BLOCK, int E, int CF>
+__device__ __forceinline__ void flush_full(int shift, uint32_t mask, const typename ScatterTup<KB>::type* s_stage,
+                                           const uint32_t* s_delta, const uint32_t* s_cur,
+                                           typename ScatterTup<KB>::type* out, uint32_t n_out, const Carry<KB>& cy,
+                                           bool last) {
+  using V = typename ScatterTup<KB>::type;
+  constexpr uint32_t LT = uint32_t(Carry<KB>::LT);
   const int tid = threadIdx.x;
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     const uint32_t j = uint32_t(e) * BLOCK + tid;
-    const typename KTr::tup_t t = s_stage[j];
-    const uint32_t d = s_delta[digit_of(KTr::low32(KTr::key(t)), shift, mask)] + j;
-    out[d < n_out ? d : n_out - 1] = t;
+    const V t = s_stage[j];
+    const uint32_t dg = digit_of(t.x, shift, mask);
+    const uint32_t p = s_delta[dg] + j;
+    V* dst = out + (p < n_out ? p : n_out - 1);
+    if constexpr (CF > 0) {
+      const uint32_t cut = last ? 0xFFFFFFFFu : (s_cur[dg] & ~(LT - 1));
+      const bool keep = p >= cut;
+      if (keep) cy.line[dg * LT + (p & (LT - 1))] = t;
+      dst = keep ? cy.sink : dst;
+    }
+    *dst = t;
   }
 }
 
 // Flush of a partial tile of len tuples (the last of a range).
 template <int KB, int BLOCK, int E>
 __device__ __forceinline__ void flush_tile(uint32_t len, int shift, uint32_t mask,
-                                           const typename KeyT<KB>::tup_t* s_stage, const uint32_t* s_delta,
-                                           typename KeyT<KB>::tup_t* out, uint32_t n_out) {
-  using KTr = KeyT<KB>;
+                                           const typename ScatterTup<KB>::type* s_stage, const uint32_t* s_delta,
+                                           typename ScatterTup<KB>::type* out, uint32_t n_out) {
   const int tid = threadIdx.x;
   for (uint32_t j = tid; j < len; j += BLOCK) {
-    const typename KTr::tup_t t = s_stage[j];
-    const uint32_t d = s_delta[digit_of(KTr::low32(KTr::key(t)), shift, mask)] + j;
+    const typename ScatterTup<KB>::type t = s_stage[j];
+    const uint32_t d = s_delta[digit_of(t.x, shift, mask)] + j;
     if (d < n_out) out[d] = t;  // always in range for exact offsets; guards the table
   }
 }
 
 // LDS layout of the scatter kernels: stage[TILE], then two count arrays,
-// base, cur and delta, F words each.
-template <int KB, int TILE>
+// base, cur and delta, F words each; with a carry also start[F] and the line
+// images [CF][128 bytes].
+template <int KB, int TILE, int CF>
 __host__ __device__ constexpr size_t scatter_lds_bytes(int F) {
-  return size_t(TILE) * (KB == 4 ? 8 : 16) + size_t(F) * 20;
+  return size_t(TILE) * (KB == 4 ? 8 : 16) + size_t(F) * (CF > 0 ? 24 : 20) + size_t(CF) * 128;
 }
 
-// Tile of a scatter pass: the large tile while the count arrays fit beside
-// it (F <= 1024), else half of it (128 KB + 20 KB or 64 KB + 80 KB of LDS).
-template <int KB>
-__host__ __device__ constexpr int scatter_tile_big() { return KB == 4 ? kScatterTile4 : kScatterTile8; }
+// Tile and carry of a scatter pass over F digits, so that stage, counts and
+// line images fit 160 KB of LDS beside pass 2's item prefix (E = TILE / 1024
+// per thread, a multiple of 4 in pass 1, whose loads are 16 bytes wide):
+//   4-byte keys  F <= 64: 16K + carry; F <= 256: 12K (pass 1) or 14K (pass 2)
+//                + carry; 512: 8K + carry; 1024: 16K; more: 8K;
+//   8-byte keys  (16-byte tuples) F <= 64: 8K + carry (pass 2), F <= 128: 8K +
+//                carry (pass 1); F <= 256: 4K (pass 1) or 7K (pass 2) + carry;
+//                512: 4K + carry; 1024: 8K; more: 4K.
+struct ScatterCfg {
+  int tile, cf;
+};
+inline ScatterCfg scatter_cfg(int key_bytes, int F, bool pass2) {
+  if (key_bytes == 4) {
+    if (F <= 64) return {16384, 64};
+    if (F <= 256) return {pass2 ? 14336 : 12288, 256};
+    if (F == 512) return {8192, 512};
+    if (F == 1024) return {16384, 0};
+    return {8192, 0};
+  }
+  if (F <= (pass2 ? 64 : 128)) return {8192, pass2 ? 64 : 128};
+  if (F <= 256) return {pass2 ? 7168 : 4096, pass2 ? 256 : 512};
+  if (F == 512) return {4096, 512};
+  if (F == 1024) return {8192, 0};
+  return {4096, 0};
+}
 
 // ---------------------------------------------------------------------------
 // k_pass1: one workgroup per segment; digit (key >> b2) & (F1 - 1).  Full
@@ -543,11 +652,11 @@ __host__ __device__ constexpr int scatter_tile_big() { return KB == 4 ? kScatter
 // under bounds checks.  PAY0 / PAY1: table 0 / 1 has a payload column (else
 // the payload is the row id).
 // ---------------------------------------------------------------------------
-template <int KB, bool VEC, bool PAY, int BLOCK, int TILE>
+template <int KB, bool VEC, bool PAY, int BLOCK, int TILE, int CF>
 __device__ __forceinline__ void pass1_body(const PartTable& T, const PartParams& pp, uint32_t b) {
   using KTr = KeyT<KB>;
   using key_t = typename KTr::key_t;
-  using tup_t = typename KTr::tup_t;
+  using tup_t = typename ScatterTup<KB>::type;  // the tuple as a vector
   constexpr int E = TILE / BLOCK;
   static_assert(E % 4 == 0, "tile must give each thread a multiple of 4 items");
   const uint32_t S = T.nseg / kNumXcd;
@@ -563,11 +672,15 @@ __device__ __forceinline__ void pass1_body(const PartTable& T, const PartParams&
   uint32_t* s_base = s_cnt_next + F;
   uint32_t* s_cur = s_base + F;
   uint32_t* s_delta = s_cur + F;
+  uint32_t* s_start = s_delta + F;
+  tup_t* out = static_cast<tup_t*>(T.buf1);
+  const Carry<KB> cy{reinterpret_cast<tup_t*>(s_start + F), s_start, out, sink_slot<KB>()};
   const int tid = threadIdx.x;
 
   const uint32_t* rel = (pp.passes == 2) ? T.matc + size_t(seg) * F : T.mat + size_t(seg) * F;
   for (int d = tid; d < F; d += BLOCK) {
     s_cur[d] = T.off[size_t(d) << pp.b2] + rel[d];
+    if (CF > 0) s_start[d] = s_cur[d];
     s_cnt[d] = 0;
   }
   lds_barrier();
@@ -576,7 +689,6 @@ __device__ __forceinline__ void pass1_body(const PartTable& T, const PartParams&
     s_cnt = s_cnt_next;
     s_cnt_next = t;
   };
-  tup_t* out = static_cast<tup_t*>(T.buf1);
   const key_t* keys = static_cast<const key_t*>(T.keys);
 
   // Item e4 * 4 + c of a thread is row tile_lo + (e4 * BLOCK + tid) * 4 + c.
@@ -639,13 +751,15 @@ __device__ __forceinline__ void pass1_body(const PartTable& T, const PartParams&
   // with fixed counts the compiler's waits for a tile's loads leave the
   // younger stores and loads in flight.
   const uint32_t nfull = (hi - lo) / uint32_t(TILE);
+  const bool no_tail = lo + nfull * uint32_t(TILE) == hi;
   // One full tile held in (ck, cp): flush the previous tile (flush_prev),
   // rank this one, issue the next tile's loads into (nk, np), scan and stage.
   // The caller alternates two register sets, so a loaded tile is never copied
   // (a copy would wait for the loads).
   auto full_tile = [&](uint32_t t, auto flush_prev, const key_t (&ck)[E], const int32_t (&cp)[E], key_t (&nk)[E],
                        int32_t (&np)[E]) {
-    if (decltype(flush_prev)::value) flush_full<KB, BLOCK, E>(shift, mask, s_stage, s_delta, out, T.n);
+    if (decltype(flush_prev)::value)
+      flush_full<KB, BLOCK, E, CF>(shift, mask, s_stage, s_delta, s_cur, out, T.n, cy, false);
     uint32_t pk[E];
     {
       uint32_t dg[E];
@@ -659,8 +773,9 @@ __device__ __forceinline__ void pass1_body(const PartTable& T, const PartParams&
     const uint32_t tile_lo = lo + t * uint32_t(TILE);
 #pragma unroll
     for (int e = 0; e < E; ++e)
-      tup[e] = KTr::make(ck[e], PAY ? cp[e] : int32_t(tile_lo + (uint32_t(e / 4) * BLOCK + tid) * 4 + (e % 4)));
-    scan_stage_tile<KB, BLOCK, E>(tup, pk, F, s_stage, s_cnt, s_cnt_next, s_base, s_cur, s_delta);
+      tup[e] = ScatterTup<KB>::make(ck[e], PAY ? cp[e] : int32_t(tile_lo + (uint32_t(e / 4) * BLOCK + tid) * 4 + (e % 4)));
+    scan_stage_tile<KB, BLOCK, E, CF>(tup, pk, F, s_stage, s_cnt, s_cnt_next, s_base, s_cur, s_delta, cy,
+                                      no_tail && t + 1 == nfull);
     swap_counts();
   };
   key_t key[E];
@@ -676,7 +791,7 @@ __device__ __forceinline__ void pass1_body(const PartTable& T, const PartParams&
       full_tile(t + 1, std::true_type{}, key, pay, key2, pay2);
     }
     if (t < nfull) full_tile(t, std::true_type{}, key2, pay2, key, pay);
-    flush_full<KB, BLOCK, E>(shift, mask, s_stage, s_delta, out, T.n);
+    flush_full<KB, BLOCK, E, CF>(shift, mask, s_stage, s_delta, s_cur, out, T.n, cy, no_tail);
   }
   // The partial last tile.
   const uint32_t tail_lo = lo + nfull * uint32_t(TILE);
@@ -687,36 +802,56 @@ __device__ __forceinline__ void pass1_body(const PartTable& T, const PartParams&
     rank_tile<E>(dg, s_cnt, pk);
     tup_t tup[E];
 #pragma unroll
-    for (int e = 0; e < E; ++e) tup[e] = KTr::make(key[e], pay[e]);
-    scan_stage_tile<KB, BLOCK, E>(tup, pk, F, s_stage, s_cnt, s_cnt_next, s_base, s_cur, s_delta);
+    for (int e = 0; e < E; ++e) tup[e] = ScatterTup<KB>::make(key[e], pay[e]);
+    scan_stage_tile<KB, BLOCK, E, CF>(tup, pk, F, s_stage, s_cnt, s_cnt_next, s_base, s_cur, s_delta, cy, true);
     flush_tile<KB, BLOCK, E>(hi - tail_lo, shift, mask, s_stage, s_delta, out, T.n);
   }
 }
 
-template <int KB, bool VEC, bool PAY0, bool PAY1, int BLOCK, int TILE>
+template <int KB, bool VEC, bool PAY0, bool PAY1, int BLOCK, int TILE, int CF>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void k_pass1(PartTable t0, PartTable t1, PartParams pp) {
   const uint32_t b = blockIdx.x;
   if (blockIdx.y == 0) {
-    if (b < t0.nseg) pass1_body<KB, VEC, PAY0, BLOCK, TILE>(t0, pp, b);
+    if (b < t0.nseg) pass1_body<KB, VEC, PAY0, BLOCK, TILE, CF>(t0, pp, b);
   } else {
-    if (b < t1.nseg) pass1_body<KB, VEC, PAY1, BLOCK, TILE>(t1, pp, b);
+    if (b < t1.nseg) pass1_body<KB, VEC, PAY1, BLOCK, TILE, CF>(t1, pp, b);
   }
 }
 
-template <int KB, bool VEC, bool PAY0, bool PAY1, int TILE>
+template <int KB, bool VEC, bool PAY0, bool PAY1, int TILE, int CF>
 static hipError_t pass1_launch_t(const Layout& L, hipStream_t s) {
   const uint32_t nseg = L.t[0].nseg > L.t[1].nseg ? L.t[0].nseg : L.t[1].nseg;
-  const size_t lds = scatter_lds_bytes<KB, TILE>(L.pp.F1);
-  hipLaunchKernelGGL((k_pass1<KB, VEC, PAY0, PAY1, kScatterBlock, TILE>), dim3(nseg, L.ntables),
+  const size_t lds = scatter_lds_bytes<KB, TILE, CF>(L.pp.F1);
+  hipLaunchKernelGGL((k_pass1<KB, VEC, PAY0, PAY1, kScatterBlock, TILE, CF>), dim3(nseg, L.ntables),
                      dim3(kScatterBlock), lds, s, L.t[0], L.t[1], L.pp);
   return hipGetLastError();
 }
 
+// Unaligned key columns (rare) take the plain tiles without a carry.
 template <int KB, bool VEC, bool PAY0, bool PAY1>
 static hipError_t pass1_launch(const Layout& L, hipStream_t s) {
-  constexpr int BIG = scatter_tile_big<KB>();
-  if (L.pp.F1 <= 1024) return pass1_launch_t<KB, VEC, PAY0, PAY1, BIG>(L, s);
-  return pass1_launch_t<KB, VEC, PAY0, PAY1, BIG / 2>(L, s);
+  constexpr int BIG = KB == 4 ? kScatterTile4 : kScatterTile8;
+  if (!VEC) {
+    if (L.pp.F1 <= 1024) return pass1_launch_t<KB, VEC, PAY0, PAY1, BIG, 0>(L, s);
+    return pass1_launch_t<KB, VEC, PAY0, PAY1, BIG / 2, 0>(L, s);
+  }
+  const ScatterCfg c = scatter_cfg(KB, L.pp.F1, false);
+#define PJ_P1(T_, C_) \
+  if (c.tile == T_ && c.cf == C_) return pass1_launch_t<KB, VEC, PAY0, PAY1, T_, C_>(L, s)
+  if constexpr (KB == 4) {
+    PJ_P1(16384, 64);
+    PJ_P1(12288, 256);
+    PJ_P1(8192, 512);
+    PJ_P1(16384, 0);
+    PJ_P1(8192, 0);
+  } else {
+    PJ_P1(8192, 128);
+    PJ_P1(4096, 512);
+    PJ_P1(8192, 0);
+    PJ_P1(4096, 0);
+  }
+#undef PJ_P1
+  return hipErrorInvalidValue;
 }
 
 template <int KB, bool VEC>
@@ -739,12 +874,10 @@ hipError_t launch_pass1(const Layout& L, int key_bytes, hipStream_t s) {
 // ---------------------------------------------------------------------------
 // k_pass2: items (p1, segment group) -> split by digit key & (F2 - 1).
 // ---------------------------------------------------------------------------
-template <int KB, int BLOCK, int TILE>
+template <int KB, int BLOCK, int TILE, int CF>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void k_pass2(PartTable t0, PartTable t1,
                                                                                         PartParams pp, uint32_t g0) {
-  using KTr = KeyT<KB>;
-  using key_t = typename KTr::key_t;
-  using tup_t = typename KTr::tup_t;
+  using tup_t = typename ScatterTup<KB>::type;  // the tuple as a vector
   constexpr int E = TILE / BLOCK;
   // Workgroups [0, g0) serve table 0 and [g0, grid) table 1, in proportion
   // to the tables' sizes; g0 is a multiple of 8, so a workgroup's XCD is its
@@ -765,13 +898,15 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
   uint32_t* s_base = s_cnt_next + F2;
   uint32_t* s_cur = s_base + F2;
   uint32_t* s_delta = s_cur + F2;
+  uint32_t* s_start = s_delta + F2;
+  tup_t* s_line = reinterpret_cast<tup_t*>(s_start + (CF > 0 ? F2 : 0));
   // Work items: pass-1 partition p1 is split into ng(p1) groups of
   // consecutive segments, ng = clamp(|p1| / (kPass2ItemTiles tiles), 1, nseg)
   // (0 for an empty p1), so an item holds about kPass2ItemTiles full tiles
   // whatever the skew.  Items are numbered in XCD-major partition order
   // r = (p1 % 8) * PX + p1 / 8; s_pre[r] counts the items before rank r.
   const uint32_t PX = uint32_t(F1 + kNumXcd - 1) / kNumXcd, RN = PX * kNumXcd;
-  uint32_t* s_pre = s_delta + F2;     // RN + 1 entries
+  uint32_t* s_pre = reinterpret_cast<uint32_t*>(s_line + CF * Carry<KB>::LT);  // RN + 1 entries
   uint32_t* s_tmp = s_pre + RN + 1;   // BLOCK / 64 + 1 entries
   const int tid = threadIdx.x;
   const uint32_t chunk = uint32_t(kPass2ItemTiles) * uint32_t(TILE);
@@ -819,12 +954,13 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
   };
   const tup_t* in = static_cast<const tup_t*>(T.buf1);
   tup_t* out = static_cast<tup_t*>(T.buf2);
+  const Carry<KB> cy{s_line, s_start, out, sink_slot<KB>()};
 
   auto digits = [&](const tup_t (&tv)[E], uint32_t tile_lo, uint32_t hi, uint32_t (&dg)[E]) {
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const bool valid = tile_lo + uint32_t(e) * BLOCK + tid < hi;
-      dg[e] = valid ? digit_of(KTr::low32(KTr::key(tv[e])), pp.shift0, mask) : 0xFFFFFFFFu;
+      dg[e] = valid ? digit_of(tv[e].x, pp.shift0, mask) : 0xFFFFFFFFu;
     }
   };
   for (uint32_t k = k_begin; k < k_end; k += uint32_t(G)) {
@@ -841,18 +977,25 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
     const uint32_t in_lo = base1 + T.matc[size_t(seg_lo) * F1 + p1];
     const uint32_t in_hi = seg_hi < nseg ? base1 + T.matc[size_t(seg_hi) * F1 + p1] : T.off[q0 + F2];
     if (in_lo >= in_hi || in_hi > T.n) continue;
-    // s_cur was last used by the previous tile's scan, before its staging barrier
-    for (int d = tid; d < F2; d += BLOCK) s_cur[d] = T.off[q0 + d] + T.mat[size_t(seg_lo) * P + q0 + d];
+    // s_cur and s_start were last read by the previous tile's scan, before
+    // its staging barrier (the flush after a range's last tile reads neither)
+    for (int d = tid; d < F2; d += BLOCK) {
+      const uint32_t c0 = T.off[q0 + d] + T.mat[size_t(seg_lo) * P + q0 + d];
+      s_cur[d] = c0;
+      if (CF > 0) s_start[d] = c0;
+    }
     // Full tiles, pipelined as in pass 1: tile 0 peeled, the next tile's
     // loads clamped to the last full tile, E stores per thread per flush.
     const uint32_t nfull = (in_hi - in_lo) / uint32_t(TILE);
+    const bool no_tail = in_lo + nfull * uint32_t(TILE) == in_hi;
     auto full_tile = [&](uint32_t t, auto flush_prev, const tup_t (&cur)[E], tup_t (&nxt)[E]) {
-      if (decltype(flush_prev)::value) flush_full<KB, BLOCK, E>(pp.shift0, mask, s_stage, s_delta, out, T.n);
+      if (decltype(flush_prev)::value)
+        flush_full<KB, BLOCK, E, CF>(pp.shift0, mask, s_stage, s_delta, s_cur, out, T.n, cy, false);
       uint32_t pk[E];
       {
         uint32_t dg[E];
 #pragma unroll
-        for (int e = 0; e < E; ++e) dg[e] = digit_of(KTr::low32(KTr::key(cur[e])), pp.shift0, mask);
+        for (int e = 0; e < E; ++e) dg[e] = digit_of(cur[e].x, pp.shift0, mask);
         rank_tile<E>(dg, s_cnt, pk);
       }
       // Next tile in flight: the next full tile, or after the last one the
@@ -861,7 +1004,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
       const uint32_t next_lo = in_lo + (t + 1) * uint32_t(TILE);
 #pragma unroll
       for (int e = 0; e < E; ++e) nxt[e] = in[min(next_lo + uint32_t(e) * BLOCK + tid, in_hi - 1)];
-      scan_stage_tile<KB, BLOCK, E>(cur, pk, F2, s_stage, s_cnt, s_cnt_next, s_base, s_cur, s_delta);
+      scan_stage_tile<KB, BLOCK, E, CF>(cur, pk, F2, s_stage, s_cnt, s_cnt_next, s_base, s_cur, s_delta, cy,
+                                        no_tail && t + 1 == nfull);
       swap_counts();
     };
     tup_t tv[E], tw[E];
@@ -875,7 +1019,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
         full_tile(t + 1, std::true_type{}, tv, tw);
       }
       if (t < nfull) full_tile(t, std::true_type{}, tw, tv);
-      flush_full<KB, BLOCK, E>(pp.shift0, mask, s_stage, s_delta, out, T.n);
+      flush_full<KB, BLOCK, E, CF>(pp.shift0, mask, s_stage, s_delta, s_cur, out, T.n, cy, no_tail);
       // the prefetched tail is in tw after an odd number of full tiles
       if (nfull & 1u) {
 #pragma unroll
@@ -895,7 +1039,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
       uint32_t dg[E], pk[E];
       digits(tv, tail_lo, in_hi, dg);
       rank_tile<E>(dg, s_cnt, pk);
-      scan_stage_tile<KB, BLOCK, E>(tv, pk, F2, s_stage, s_cnt, s_cnt_next, s_base, s_cur, s_delta);
+      scan_stage_tile<KB, BLOCK, E, CF>(tv, pk, F2, s_stage, s_cnt, s_cnt_next, s_base, s_cur, s_delta, cy, true);
       swap_counts();
       flush_tile<KB, BLOCK, E>(in_hi - tail_lo, pp.shift0, mask, s_stage, s_delta, out, T.n);
     }
@@ -916,22 +1060,31 @@ hipError_t launch_pass2(const Layout& L, int key_bytes, hipStream_t s) {
   }
   const uint32_t g0 = x0 * uint32_t(kNumXcd);
   dim3 grid(per_xcd * kNumXcd), block(kScatterBlock);
-  const bool big = L.pp.F2 <= 1024;
   // the item prefix (one entry per pass-1 partition rounded up to 8, plus one)
   // and the scan's temporaries follow the scatter's LDS
   const size_t extra = (size_t((L.pp.F1 + kNumXcd - 1) / kNumXcd) * kNumXcd + 1 + kScatterBlock / kWave + 1) * 4;
-#define PJ_PASS2(KB_, TILE_)                                                                                     \
-  hipLaunchKernelGGL((k_pass2<KB_, kScatterBlock, TILE_>), grid, block, (scatter_lds_bytes<KB_, TILE_>(L.pp.F2) + extra), s, \
-                     L.t[0], L.t[1], L.pp, g0)
+  const ScatterCfg c = scatter_cfg(key_bytes, L.pp.F2, true);
+#define PJ_PASS2(KB_, TILE_, CF_)                                                                                   \
+  if (c.tile == TILE_ && c.cf == CF_) {                                                                             \
+    hipLaunchKernelGGL((k_pass2<KB_, kScatterBlock, TILE_, CF_>), grid, block,                                     \
+                       (scatter_lds_bytes<KB_, TILE_, CF_>(L.pp.F2) + extra), s, L.t[0], L.t[1], L.pp, g0);        \
+    return hipGetLastError();                                                                                       \
+  }
   if (key_bytes == 4) {
-    if (big) PJ_PASS2(4, kScatterTile4);
-    else PJ_PASS2(4, kScatterTile4 / 2);
+    PJ_PASS2(4, 16384, 64)
+    PJ_PASS2(4, 14336, 256)
+    PJ_PASS2(4, 8192, 512)
+    PJ_PASS2(4, 16384, 0)
+    PJ_PASS2(4, 8192, 0)
   } else {
-    if (big) PJ_PASS2(8, kScatterTile8);
-    else PJ_PASS2(8, kScatterTile8 / 2);
+    PJ_PASS2(8, 8192, 64)
+    PJ_PASS2(8, 7168, 256)
+    PJ_PASS2(8, 4096, 512)
+    PJ_PASS2(8, 8192, 0)
+    PJ_PASS2(8, 4096, 0)
   }
 #undef PJ_PASS2
-  return hipGetLastError();
+  return hipErrorInvalidValue;
 }
 
 }  // namespace pj
