@@ -494,14 +494,24 @@ __device__ __forceinline__ typename ScatterTup<KB>::type* sink_slot() {
   return reinterpret_cast<typename ScatterTup<KB>::type*>(g_scatter_sink + size_t(wg) * kWave + lane_id());
 }
 
+// The carried lines that a tile completes are read into registers at the end
+// of the tile's staging and stored by the tile's flush, together with the
+// rest of their lines: written apart, a half line could leave the L2 before
+// its other half arrives and reach HBM as a partial line after all.
+template <int KB, int BLOCK, int CF>
+struct CarryRegs {
+  static constexpr int NR = CF > 0 ? (CF * Carry<KB>::LT + BLOCK - 1) / BLOCK : 1;
+  typename Carry<KB>::vec_t v[NR];
+  uint32_t pos[NR];  // destination row, or 0xFFFFFFFF: nothing to store
+};
+
 // After a tile's scan (s_cur advanced by the tile's counts s_cnt): the carried
 // line of each digit whose run the tile completes (or every one, for the last
-// tile) goes out -- its positions from the range start or the line start,
-// whichever is later, up to the old cursor.  ceil(CF * LT / BLOCK) stores per
-// thread.
+// tile) -- its positions from the range start or the line start, whichever
+// is later, up to the old cursor.
 template <int KB, int BLOCK, int CF>
-__device__ __forceinline__ void carry_out(int F, const uint32_t* s_cnt, const uint32_t* s_cur, const Carry<KB>& cy,
-                                          bool last) {
+__device__ __forceinline__ void carry_read(int F, const uint32_t* s_cnt, const uint32_t* s_cur, const Carry<KB>& cy,
+                                           bool last, CarryRegs<KB, BLOCK, CF>& cr) {
   constexpr uint32_t LT = uint32_t(Carry<KB>::LT), NS = uint32_t(CF) * LT;
   // The thread index through an opaque move: the slot addresses below are
   // then recomputed per tile (a few adds) instead of being hoisted out of
@@ -510,15 +520,23 @@ __device__ __forceinline__ void carry_out(int F, const uint32_t* s_cnt, const ui
   uint32_t tid;
   __asm__ volatile("v_mov_b32 %0, %1" : "=v"(tid) : "v"(uint32_t(threadIdx.x)));
 #pragma unroll
-  for (int r = 0; r < int((NS + BLOCK - 1) / BLOCK); ++r) {
+  for (int r = 0; r < CarryRegs<KB, BLOCK, CF>::NR; ++r) {
     const uint32_t slot = min(uint32_t(r) * BLOCK + tid, NS - 1), d = slot / LT, k = slot % LT;
     const uint32_t dd = d < uint32_t(F) ? d : uint32_t(F) - 1;
     const uint32_t cur = s_cur[dd], old = cur - s_cnt[dd], l0 = old & ~(LT - 1), pos = l0 + k;
     const bool done = last || (cur & ~(LT - 1)) > l0;
     const bool valid = uint32_t(r) * BLOCK + tid < NS && d < uint32_t(F) && done && pos < old && pos >= cy.start[dd];
-    const typename Carry<KB>::vec_t v = cy.line[slot];
-    *(valid ? cy.out + pos : cy.sink) = v;
+    cr.v[r] = cy.line[slot];
+    cr.pos[r] = valid ? pos : 0xFFFFFFFFu;
   }
+}
+
+// NR stores per thread, unconditional (the sink takes the empty ones).
+template <int KB, int BLOCK, int CF>
+__device__ __forceinline__ void carry_write(const Carry<KB>& cy, const CarryRegs<KB, BLOCK, CF>& cr) {
+#pragma unroll
+  for (int r = 0; r < CarryRegs<KB, BLOCK, CF>::NR; ++r)
+    *(cr.pos[r] != 0xFFFFFFFFu ? cy.out + cr.pos[r] : cy.sink) = cr.v[r];
 }
 
 // After rank_tile: the first wave scans the F digit counts (F / 64 per lane,
@@ -530,7 +548,8 @@ template <int KB, int BLOCK, int E, int CF>
 __device__ __forceinline__ void scan_stage_tile(const typename ScatterTup<KB>::type (&tup)[E], const uint32_t (&pk)[E],
                                                 int F, typename ScatterTup<KB>::type* s_stage, const uint32_t* s_cnt,
                                                 uint32_t* s_cnt_next, uint32_t* s_base, uint32_t* s_cur,
-                                                uint32_t* s_delta, const Carry<KB>& cy, bool last) {
+                                                uint32_t* s_delta, const Carry<KB>& cy, bool last,
+                                                CarryRegs<KB, BLOCK, CF>& cr) {
   const int tid = threadIdx.x;
   lds_barrier();
   if (tid < kWave) {
@@ -558,11 +577,11 @@ __device__ __forceinline__ void scan_stage_tile(const typename ScatterTup<KB>::t
     }
   }
   lds_barrier();
-  if constexpr (CF > 0) carry_out<KB, BLOCK, CF>(F, s_cnt, s_cur, cy, last);
   for (int d = tid; d < F; d += BLOCK) s_cnt_next[d] = 0;
 #pragma unroll
   for (int e = 0; e < E; ++e)
     if (pk[e] != 0xFFFFFFFFu) s_stage[s_base[pk[e] >> 16] + (pk[e] & 0xFFFFu)] = tup[e];
+  if constexpr (CF > 0) carry_read<KB, BLOCK, CF>(F, s_cnt, s_cur, cy, last, cr);
   lds_barrier();
 }
 
@@ -577,7 +596,7 @@ template <int KB, int BLOCK, int E, int CF>
 __device__ __forceinline__ void flush_full(int shift, uint32_t mask, const typename ScatterTup<KB>::type* s_stage,
                                            const uint32_t* s_delta, const uint32_t* s_cur,
                                            typename ScatterTup<KB>::type* out, uint32_t n_out, const Carry<KB>& cy,
-                                           bool last) {
+                                           bool last, const CarryRegs<KB, BLOCK, CF>& cr) {
   using V = typename ScatterTup<KB>::type;
   constexpr uint32_t LT = uint32_t(Carry<KB>::LT);
   const int tid = threadIdx.x;
@@ -596,19 +615,23 @@ __device__ __forceinline__ void flush_full(int shift, uint32_t mask, const typen
     }
     *dst = t;
   }
+  if constexpr (CF > 0) carry_write<KB, BLOCK, CF>(cy, cr);
 }
 
-// Flush of a partial tile of len tuples (the last of a range).
-template <int KB, int BLOCK, int E>
+// Flush of a partial tile of len tuples (the last of a range, so nothing is
+// carried), with the carried lines the tile's staging read.
+template <int KB, int BLOCK, int E, int CF>
 __device__ __forceinline__ void flush_tile(uint32_t len, int shift, uint32_t mask,
                                            const typename ScatterTup<KB>::type* s_stage, const uint32_t* s_delta,
-                                           typename ScatterTup<KB>::type* out, uint32_t n_out) {
+                                           typename ScatterTup<KB>::type* out, uint32_t n_out, const Carry<KB>& cy,
+                                           const CarryRegs<KB, BLOCK, CF>& cr) {
   const int tid = threadIdx.x;
   for (uint32_t j = tid; j < len; j += BLOCK) {
     const typename ScatterTup<KB>::type t = s_stage[j];
     const uint32_t d = s_delta[digit_of(t.x, shift, mask)] + j;
     if (d < n_out) out[d] = t;  // always in range for exact offsets; guards the table
   }
+  if constexpr (CF > 0) carry_write<KB, BLOCK, CF>(cy, cr);
 }
 
 // LDS layout of the scatter kernels: stage[TILE], then two count arrays,
@@ -752,6 +775,7 @@ __device__ __forceinline__ void pass1_body(const PartTable& T, const PartParams&
   // younger stores and loads in flight.
   const uint32_t nfull = (hi - lo) / uint32_t(TILE);
   const bool no_tail = lo + nfull * uint32_t(TILE) == hi;
+  CarryRegs<KB, BLOCK, CF> cr;  // carried lines read by a tile's staging, stored by its flush
   // One full tile held in (ck, cp): flush the previous tile (flush_prev),
   // rank this one, issue the next tile's loads into (nk, np), scan and stage.
   // The caller alternates two register sets, so a loaded tile is never copied
@@ -759,7 +783,7 @@ __device__ __forceinline__ void pass1_body(const PartTable& T, const PartParams&
   auto full_tile = [&](uint32_t t, auto flush_prev, const key_t (&ck)[E], const int32_t (&cp)[E], key_t (&nk)[E],
                        int32_t (&np)[E]) {
     if (decltype(flush_prev)::value)
-      flush_full<KB, BLOCK, E, CF>(shift, mask, s_stage, s_delta, s_cur, out, T.n, cy, false);
+      flush_full<KB, BLOCK, E, CF>(shift, mask, s_stage, s_delta, s_cur, out, T.n, cy, false, cr);
     uint32_t pk[E];
     {
       uint32_t dg[E];
@@ -775,7 +799,7 @@ __device__ __forceinline__ void pass1_body(const PartTable& T, const PartParams&
     for (int e = 0; e < E; ++e)
       tup[e] = ScatterTup<KB>::make(ck[e], PAY ? cp[e] : int32_t(tile_lo + (uint32_t(e / 4) * BLOCK + tid) * 4 + (e % 4)));
     scan_stage_tile<KB, BLOCK, E, CF>(tup, pk, F, s_stage, s_cnt, s_cnt_next, s_base, s_cur, s_delta, cy,
-                                      no_tail && t + 1 == nfull);
+                                      no_tail && t + 1 == nfull, cr);
     swap_counts();
   };
   key_t key[E];
@@ -791,7 +815,7 @@ __device__ __forceinline__ void pass1_body(const PartTable& T, const PartParams&
       full_tile(t + 1, std::true_type{}, key, pay, key2, pay2);
     }
     if (t < nfull) full_tile(t, std::true_type{}, key2, pay2, key, pay);
-    flush_full<KB, BLOCK, E, CF>(shift, mask, s_stage, s_delta, s_cur, out, T.n, cy, no_tail);
+    flush_full<KB, BLOCK, E, CF>(shift, mask, s_stage, s_delta, s_cur, out, T.n, cy, no_tail, cr);
   }
   // The partial last tile.
   const uint32_t tail_lo = lo + nfull * uint32_t(TILE);
@@ -803,8 +827,8 @@ __device__ __forceinline__ void pass1_body(const PartTable& T, const PartParams&
     tup_t tup[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) tup[e] = ScatterTup<KB>::make(key[e], pay[e]);
-    scan_stage_tile<KB, BLOCK, E, CF>(tup, pk, F, s_stage, s_cnt, s_cnt_next, s_base, s_cur, s_delta, cy, true);
-    flush_tile<KB, BLOCK, E>(hi - tail_lo, shift, mask, s_stage, s_delta, out, T.n);
+    scan_stage_tile<KB, BLOCK, E, CF>(tup, pk, F, s_stage, s_cnt, s_cnt_next, s_base, s_cur, s_delta, cy, true, cr);
+    flush_tile<KB, BLOCK, E, CF>(hi - tail_lo, shift, mask, s_stage, s_delta, out, T.n, cy, cr);
   }
 }
 
@@ -988,9 +1012,10 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
     // loads clamped to the last full tile, E stores per thread per flush.
     const uint32_t nfull = (in_hi - in_lo) / uint32_t(TILE);
     const bool no_tail = in_lo + nfull * uint32_t(TILE) == in_hi;
+    CarryRegs<KB, BLOCK, CF> cr;  // carried lines read by a tile's staging, stored by its flush
     auto full_tile = [&](uint32_t t, auto flush_prev, const tup_t (&cur)[E], tup_t (&nxt)[E]) {
       if (decltype(flush_prev)::value)
-        flush_full<KB, BLOCK, E, CF>(pp.shift0, mask, s_stage, s_delta, s_cur, out, T.n, cy, false);
+        flush_full<KB, BLOCK, E, CF>(pp.shift0, mask, s_stage, s_delta, s_cur, out, T.n, cy, false, cr);
       uint32_t pk[E];
       {
         uint32_t dg[E];
@@ -1005,7 +1030,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
 #pragma unroll
       for (int e = 0; e < E; ++e) nxt[e] = in[min(next_lo + uint32_t(e) * BLOCK + tid, in_hi - 1)];
       scan_stage_tile<KB, BLOCK, E, CF>(cur, pk, F2, s_stage, s_cnt, s_cnt_next, s_base, s_cur, s_delta, cy,
-                                        no_tail && t + 1 == nfull);
+                                        no_tail && t + 1 == nfull, cr);
       swap_counts();
     };
     tup_t tv[E], tw[E];
@@ -1019,7 +1044,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
         full_tile(t + 1, std::true_type{}, tv, tw);
       }
       if (t < nfull) full_tile(t, std::true_type{}, tw, tv);
-      flush_full<KB, BLOCK, E, CF>(pp.shift0, mask, s_stage, s_delta, s_cur, out, T.n, cy, no_tail);
+      flush_full<KB, BLOCK, E, CF>(pp.shift0, mask, s_stage, s_delta, s_cur, out, T.n, cy, no_tail, cr);
       // the prefetched tail is in tw after an odd number of full tiles
       if (nfull & 1u) {
 #pragma unroll
@@ -1039,9 +1064,9 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
       uint32_t dg[E], pk[E];
       digits(tv, tail_lo, in_hi, dg);
       rank_tile<E>(dg, s_cnt, pk);
-      scan_stage_tile<KB, BLOCK, E, CF>(tv, pk, F2, s_stage, s_cnt, s_cnt_next, s_base, s_cur, s_delta, cy, true);
+      scan_stage_tile<KB, BLOCK, E, CF>(tv, pk, F2, s_stage, s_cnt, s_cnt_next, s_base, s_cur, s_delta, cy, true, cr);
       swap_counts();
-      flush_tile<KB, BLOCK, E>(in_hi - tail_lo, pp.shift0, mask, s_stage, s_delta, out, T.n);
+      flush_tile<KB, BLOCK, E, CF>(in_hi - tail_lo, pp.shift0, mask, s_stage, s_delta, out, T.n, cy, cr);
     }
   }
 }
