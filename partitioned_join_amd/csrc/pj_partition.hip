@@ -118,10 +118,11 @@ template <int KB, bool PACKED, bool VEC>
 __global__ __launch_bounds__(kHistBlock) void k_hist(PartTable t0, PartTable t1, PartParams pp) {
   constexpr int BLOCK = kHistBlock;
   // Rounds of 4 keys per thread loaded together before any is counted, so
-  // each wave keeps BATCH loads in flight (one round at a time leaves the
-  // loop latency-bound at a fraction of HBM bandwidth).
-  constexpr uint32_t BATCH = 5;
-  constexpr uint32_t kEpochBatches = 3;  // packed bins: 3 * 5 * 4096 = 61440 keys < 65536 per epoch
+  // each wave keeps BATCH loads in flight, and the next batch is loaded while
+  // the current one is counted (two register sets; one round at a time
+  // leaves the loop latency-bound at a fraction of HBM bandwidth).
+  constexpr uint32_t BATCH = 4;
+  constexpr uint32_t kEpochBatches = 3;  // packed bins: 3 * 4 * 4096 = 49152 keys < 65536 per epoch
   const PartTable T = blockIdx.y ? t1 : t0;
   const uint32_t b = blockIdx.x;
   if (b >= T.nseg) return;
@@ -146,8 +147,7 @@ __global__ __launch_bounds__(kHistBlock) void k_hist(PartTable t0, PartTable t1,
     else atomicAdd(&s_bins[d], 1u);
   };
   // Batch f: rounds f * BATCH .. f * BATCH + BATCH - 1, all in range.
-  auto count_batch = [&](uint32_t f) {
-    uint32_t low[BATCH][4];
+  auto load_batch = [&](uint32_t f, uint32_t (&low)[BATCH][4]) {
 #pragma unroll
     for (uint32_t u = 0; u < BATCH; ++u) {
       const uint32_t i = lo + ((f * BATCH + u) * BLOCK + tid) * 4;
@@ -161,10 +161,17 @@ __global__ __launch_bounds__(kHistBlock) void k_hist(PartTable t0, PartTable t1,
         low[u][0] = a.x; low[u][1] = a.z; low[u][2] = c.x; low[u][3] = c.z;
       }
     }
+  };
+  auto count_low = [&](const uint32_t (&low)[BATCH][4]) {
 #pragma unroll
     for (uint32_t u = 0; u < BATCH; ++u)
 #pragma unroll
       for (int c = 0; c < 4; ++c) count(low[u][c]);
+  };
+  auto count_batch = [&](uint32_t f) {
+    uint32_t low[BATCH][4];
+    load_batch(f, low);
+    count_low(low);
   };
   auto count_round = [&](uint32_t r) {
     uint32_t low[4];
@@ -211,9 +218,23 @@ __global__ __launch_bounds__(kHistBlock) void k_hist(PartTable t0, PartTable t1,
       }
       lds_barrier();
     };
-    for (uint32_t f0 = 0; f0 < nfull; f0 += kEpochBatches) {
-      const uint32_t f1 = min(nfull, f0 + kEpochBatches);
-      for (uint32_t f = f0; f < f1; ++f) count_batch(f);
+    // Full batches, pipelined: batch f + 1 loads while batch f is counted
+    // (the last iteration reloads the last batch rather than branch, so the
+    // loads have fixed counts and the waits stay exact); a fold closes every
+    // kEpochBatches batches and the last one.
+    if (nfull > 0) {
+      uint32_t la[BATCH][4], lb[BATCH][4];
+      load_batch(0, la);
+      uint32_t f = 0;
+      for (; f + 1 < nfull; f += 2) {
+        load_batch(f + 1, lb);
+        count_low(la);
+        if ((f + 1) % kEpochBatches == 0) fold();
+        load_batch(min(f + 2, nfull - 1), la);
+        count_low(lb);
+        if ((f + 2) % kEpochBatches == 0) fold();
+      }
+      if (f < nfull) count_low(la);
       fold();
     }
     constexpr uint32_t kEpochRounds = kEpochBatches * BATCH;
