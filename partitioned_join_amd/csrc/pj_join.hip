@@ -495,7 +495,11 @@ __global__ __launch_bounds__(BLOCK) void k_join(JoinArgs A) {
     // the next item may overwrite the table, and the stage is copied out after
     // the next item's first barrier.
     if (tid == 0) plan_copy<STAGE>(sink, &ctl, ctl.n);
-    // advance the pipeline
+    // advance the pipeline.  The empty asm keeps the raw descriptor words in
+    // VGPRs until here: the compiler would otherwise move them to scalar
+    // registers right after their loads, which waits for every load issued
+    // before them -- the next item's tuples included.
+    __asm__ volatile("" : "+v"(ra.x), "+v"(ra.y), "+v"(ra.z), "+v"(ra.w), "+v"(rb.x), "+v"(rb.y), "+v"(rb.z), "+v"(rb.w));
     d = dn;
     dn = desc_from_raw(A, ra, rb, item_nn, total_items);
 #pragma unroll
