@@ -193,6 +193,12 @@ __global__ __launch_bounds__(256) void k_items_expand(JoinArgs A, uint4* desc, u
   }
 }
 
+// An empty asm that takes a tuple's registers as read and rewritten.
+__device__ __forceinline__ void tup_pin(uint2& t) { __asm__ volatile("" : "+v"(t.x), "+v"(t.y)); }
+__device__ __forceinline__ void tup_pin(Tup8& t) {
+  __asm__ volatile("" : "+v"(t.x), "+v"(t.y), "+v"(t.z), "+v"(t.w));
+}
+
 // Raw descriptor words of `item`, loaded without a branch (the index is
 // clamped) so that nothing consumes them before desc_from_raw(): the loads
 // stay in flight across the current item.
@@ -247,7 +253,7 @@ __device__ __forceinline__ ItemDesc desc_from_raw(const JoinArgs& A, const uint4
 // unwritten matches pending, the workgroup flushes and the round resumes.
 // ---------------------------------------------------------------------------
 template <int KB, int BLOCK, int NB, int R, bool UNIQUE>
-__global__ __launch_bounds__(BLOCK) void k_join(JoinArgs A) {
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void k_join(JoinArgs A) {
   using KTr = KeyT<KB>;
   using key_t = typename KTr::key_t;
   using tup_t = typename KTr::tup_t;
@@ -302,6 +308,14 @@ __global__ __launch_bounds__(BLOCK) void k_join(JoinArgs A) {
   if (d.valid) load_item(d, bt, pt);
 
   while (d.valid) {
+    // The empty asm redefines this item's tuples (copied from the previous
+    // prefetch) before the next prefetch is issued: the compiler's wait for
+    // them then falls here, and not in the build, where it would also wait
+    // for the next item's loads.
+#pragma unroll
+    for (int k = 0; k < NB; ++k) tup_pin(bt[k]);
+#pragma unroll
+    for (int k = 0; k < R; ++k) tup_pin(pt[k]);
     tup_t bn[NB], pn[R];
     if (dn.valid) load_item(dn, bn, pn);
     const uint64_t item_nn = item + 2 * uint64_t(gridDim.x);
