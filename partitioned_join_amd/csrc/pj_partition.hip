@@ -668,9 +668,7 @@ __host__ __device__ constexpr size_t scatter_lds_bytes(int F) {
 // per thread, a multiple of 4 in pass 1, whose loads are 16 bytes wide):
 //   4-byte keys  F <= 64: 16K + carry; F <= 256: 12K (pass 1) or 14K (pass 2)
 //                + carry; 512: 8K + carry; 1024: 16K; more: 8K;
-//   8-byte keys  (16-byte tuples) F <= 64: 8K + carry (pass 2), F <= 128: 8K +
-//                carry (pass 1); F <= 256: 4K (pass 1) or 7K (pass 2) + carry;
-//                512: 4K + carry; 1024: 8K; more: 4K.
+//   8-byte keys  (16-byte tuples) F <= 1024: 8K; more: 4K, without a carry.
 struct ScatterCfg {
   int tile, cf;
 };
@@ -682,11 +680,10 @@ inline ScatterCfg scatter_cfg(int key_bytes, int F, bool pass2) {
     if (F == 1024) return {16384, 0};
     return {8192, 0};
   }
-  if (F <= (pass2 ? 64 : 128)) return {8192, pass2 ? 64 : 128};
-  if (F <= 256) return {pass2 ? 7168 : 4096, pass2 ? 256 : 512};
-  if (F == 512) return {4096, 512};
-  if (F == 1024) return {8192, 0};
-  return {4096, 0};
+  // 16-byte tuples: the carry variants keep their register arrays in scratch
+  // memory (measured slower on 64-bit-key joins), so they scatter without one
+  (void)pass2;
+  return {F <= 1024 ? 8192 : 4096, 0};
 }
 
 // ---------------------------------------------------------------------------
@@ -890,8 +887,6 @@ static hipError_t pass1_launch(const Layout& L, hipStream_t s) {
     PJ_P1(16384, 0);
     PJ_P1(8192, 0);
   } else {
-    PJ_P1(8192, 128);
-    PJ_P1(4096, 512);
     PJ_P1(8192, 0);
     PJ_P1(4096, 0);
   }
@@ -1123,9 +1118,6 @@ hipError_t launch_pass2(const Layout& L, int key_bytes, hipStream_t s) {
     PJ_PASS2(4, 16384, 0)
     PJ_PASS2(4, 8192, 0)
   } else {
-    PJ_PASS2(8, 8192, 64)
-    PJ_PASS2(8, 7168, 256)
-    PJ_PASS2(8, 4096, 512)
     PJ_PASS2(8, 8192, 0)
     PJ_PASS2(8, 4096, 0)
   }
