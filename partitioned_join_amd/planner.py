@@ -14,9 +14,11 @@ It predicts three things:
 - for each of the 64 most frequent sampled keys, whether it holds at least
   1/1024 of the probe rows.
 
-The library takes the learned plan only when that predicted heavy set equals
-the exact one, counted over the whole probe column. Otherwise it returns the
-plain histogram-driven plan. Either way the join result is the same.
+The library takes the learned plan only when that predicted heavy set agrees
+with the exact candidate counts over the whole probe column, within a band: a
+key predicted heavy must hold at least T/2 rows and a key predicted light
+fewer than 2T, for T = |S|/1024. Otherwise it returns the plain
+histogram-driven plan. Either way the join result is the same.
 
 Training data follows the planner training/eval workload: seeded joins with
 |R| log-uniform in [1M, 64M], |S|/|R| in [1, 16], int32 or int64 keys, Zipf

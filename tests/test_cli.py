@@ -1,0 +1,81 @@
+"""The `partitioned_join` command-line driver: the reference binary's flags
+(main.cu:100-120) and CSV columns (scripts/bench_partitioned_join.sh:12).
+
+Argument errors are checked on the CPU (they are rejected before any device
+call).  On the GPU every generator mode and the binary key-file input run
+small joins, and the driver's own CPU check (the reference's check_results,
+src/inner_join_cpu.hpp:109-168) must report zero errors in the last CSV
+column."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CLI = os.path.join(ROOT, "partitioned_join_amd", "partitioned_join")
+
+# build, probe, key bytes, tuple bytes, selectivity, unique, P1, P2, factor,
+# skew mode, zipf, joined, histogram ms, P1 ms, P2 ms, skew ms, join ms, total ms, errors
+CSV_FIELDS = 19
+
+
+def _run(args, timeout=120):
+    assert os.path.exists(CLI), "build the driver first (make -C partitioned_join_amd)"
+    return subprocess.run([CLI, *args], capture_output=True, text=True, timeout=timeout)
+
+
+def test_help_and_argument_errors():
+    r = _run(["-h"])
+    assert r.returncode == 1 and "Usage: partitioned_join" in r.stdout
+    r = _run(["-b", "1000", "-n", "3"])
+    assert r.returncode == 1 and "power of two" in r.stderr
+    r = _run(["-R", "-n", "16", "-N", "12"])
+    assert r.returncode == 1 and "power of two" in r.stderr
+    r = _run(["-K", "5"])
+    assert r.returncode == 1 and "-K must be 4 or 8" in r.stderr
+    r = _run(["-b", str(1 << 31)])
+    assert r.returncode == 1 and "table sizes" in r.stderr
+    r = _run(["--no-such-flag"])
+    assert r.returncode == 1
+
+
+def _csv(args):
+    r = _run(["-C", *args])
+    assert r.returncode == 0, (args, r.stdout[-2000:], r.stderr[-2000:])
+    line = r.stdout.strip().splitlines()[-1].split(",")
+    assert len(line) == CSV_FIELDS, line
+    return line
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("args", [
+    ["-b", "100000", "-p", "400000"],                                   # one pass, duplicate build keys
+    ["-b", "100000", "-p", "400000", "-u", "-s", "0.5"],                # unique build keys, lottery probes
+    ["-b", "200000", "-p", "800000", "-R", "-n", "16", "-N", "32"],     # two passes
+    ["-b", "100000", "-p", "300000", "-V", "-R", "-n", "8", "-N", "8"],  # payload columns
+    ["-b", "50000", "-p", "200000", "-Z", "2", "-z", "1.0"],            # Zipf probes over unique keys
+    ["-b", "50000", "-p", "200000", "-Z", "1"],                         # skew mode 1: full selectivity
+    ["-b", "100000", "-p", "400000", "-K", "8", "-s", "0.7"],           # 64-bit keys
+])
+def test_generated_inputs_match_cpu_check(args):
+    f = _csv(args)
+    assert f[-1] == "0", f
+    assert int(f[11]) >= 0 and float(f[17]) > 0
+
+
+@pytest.mark.gpu
+def test_key_files(tmp_path):
+    rng = np.random.default_rng(5)
+    build = rng.permutation(1 << 17).astype(np.int32)
+    probe = rng.integers(0, 1 << 18, 1 << 19).astype(np.int32)
+    bf, pf = tmp_path / "build.bin", tmp_path / "probe.bin"
+    build.tofile(bf)
+    probe.tofile(pf)
+    f = _csv(["-F", "-B", str(bf), "-P", str(pf), "-b", str(build.size), "-p", str(probe.size), "-R", "-n", "16",
+              "-N", "16"])
+    assert f[-1] == "0"
+    assert int(f[11]) == int(np.count_nonzero(probe < (1 << 17)))
+    # a file shorter than the requested size is an error, not a short read
+    r = _run(["-C", "-F", "-B", str(bf), "-P", str(pf), "-b", str(build.size + 1), "-p", str(probe.size)])
+    assert r.returncode == 1 and "fewer than" in r.stderr
