@@ -24,7 +24,7 @@ constexpr int kPass2WgPerXcd = 16;    // pass-2 persistent workgroups per XCD an
 constexpr int kPass2ItemTiles = 4;    // pass-2 items hold about this many full tiles
 constexpr int kMaxTotalBits = 16;     // combined histogram limit (65536 partitions)
 constexpr int kMaxRadix = 4096;       // largest fan-out of one pass (12 bits)
-constexpr int kMaxSegPerXcd = 64;
+constexpr int kMaxSegPerXcd = 32;    // segments per XCD and table at most: the [segment][digit] matrix the histogram writes and the column scans read shrinks with it, while pass 1 (one workgroup per segment) still has two per CU (64M x 256M: 64 -> 4.41 ms, 32 -> 4.17, 16 -> 4.39)
 
 // One table being partitioned.  All pointers are device pointers.
 struct PartTable {
