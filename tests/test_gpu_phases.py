@@ -119,7 +119,8 @@ def tuples_np(t, key_bytes):
 
 @pytest.mark.parametrize("n,shift,bits,with_pay", [(0, 0, 4, False), (1, 0, 1, False), (100, 0, 3, True),
                                                    (65536 + 17, 0, 8, False), (1 << 20, 4, 10, True),
-                                                   (3000001, 0, 12, False), (2000000, 9, 7, True)])
+                                                   (3000001, 0, 12, False), (2000000, 9, 7, True),
+                                                   (1500007, 0, 9, True)])
 def test_partition_pass_one(pj, oracle, torch, n, shift, bits, with_pay):
     rng = np.random.default_rng(n + bits)
     keys = rng.integers(-2**31, 2**31 - 1, n).astype(np.int32)
@@ -149,7 +150,9 @@ def test_partition_pass_one_skewed(pj, oracle, torch):
     oracle.check_partitioned(keys, None, k, p, off.cpu().numpy().astype(np.int64), 0, 8)
 
 
-@pytest.mark.parametrize("n,b1,b2,kb", [(1000, 3, 2, 4), (1 << 20, 8, 8, 4), (777777, 6, 9, 4), (300000, 5, 5, 8)])
+# pass-2 fan-outs 4..4096 cover every scatter tile / line-carry configuration
+@pytest.mark.parametrize("n,b1,b2,kb", [(1000, 3, 2, 4), (1 << 20, 8, 8, 4), (777777, 6, 9, 4), (300000, 5, 5, 8),
+                                        (900001, 4, 10, 4), (1 << 20, 3, 12, 4), (500009, 2, 11, 8)])
 def test_partition_pass_two(pj, oracle, torch, n, b1, b2, kb):
     rng = np.random.default_rng(n)
     dt = np.int32 if kb == 4 else np.int64
