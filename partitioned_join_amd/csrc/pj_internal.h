@@ -12,7 +12,7 @@ namespace pj {
 // Tuning constants (gfx950: 256 CUs, 160 KB LDS per CU, wave64).
 constexpr int kHistBlock = 1024;      // combined histogram workgroup
 constexpr int kScatterBlock = 1024;   // partition pass workgroup (one per CU: 16 waves)
-constexpr int kScatterTile4 = 16384;  // tuples staged in LDS per tile, 4-byte keys (128 KB)
+constexpr int kScatterTile4 = 16384;  // largest scatter tile, 4-byte keys (128 KB of LDS; scatter_cfg picks the tile per fan-out)
 constexpr int kScatterTile8 = 8192;   // 8-byte keys (128 KB)
 constexpr int kJoinBlock = 512;       // join workgroup
 constexpr int kJoinProbe4 = 9;        // probe tuples per thread and round, 4-byte keys (4608 per round)

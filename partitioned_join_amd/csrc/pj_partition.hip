@@ -15,13 +15,15 @@
 //              (cub::DeviceScan::ExclusiveSum, src/partitioned_join.cuh:86,152).
 //   k_pass1    one workgroup per segment: tiles of the segment are ranked by
 //              digit with LDS atomics, reordered through an LDS staging tile and
-//              written as coalesced runs to the workgroup's private, contiguous
+//              written as whole 128-byte lines (unfinished lines are carried in
+//              LDS to the next tile) to the workgroup's private, contiguous
 //              range of every partition (partition_pass_one, :196-326).
 //   k_pass2    pass-1 partition p1 is split at segment-group boundaries into
 //              items whose per-digit output starts are known from the matrix;
-//              all items of one p1 run on one XCD so partial lines merge in its
-//              L2 (partition_pass_two, :408-563 with handle_skewed_data's block
-//              assignment, src/skewed_data_handler.cuh:58-91).
+//              the number of groups follows p1's size (skew), and the items are
+//              cut into one equal run per XCD (partition_pass_two, :408-563
+//              with handle_skewed_data's block assignment,
+//              src/skewed_data_handler.cuh:58-91).
 // Segment s of a table is processed on XCD s / (nseg/8) by both the histogram
 // and pass 1 (workgroup b runs on XCD b % 8).
 #include <cmath>
