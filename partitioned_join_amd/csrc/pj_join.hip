@@ -619,32 +619,18 @@ __device__ __forceinline__ uint32_t upper_index(const uint64_t* pre, uint32_t n,
   return lo;
 }
 
-// The hole and stray tables (at most kJoinGrid + 1 entries each) are staged in
-// LDS, so each move's two binary searches read LDS instead of making ~20
-// dependent global loads.
 __global__ __launch_bounds__(256) void k_defrag_move(const DefragState* st, const uint64_t* hole_lo,
                                                      const uint64_t* hole_pre, const uint64_t* stray_lo,
                                                      const uint64_t* stray_pre, uint2* out, int64_t capacity,
                                                      const uint2* spill) {
-  __shared__ uint64_t s_hlo[kJoinGrid + 1], s_hpre[kJoinGrid + 1], s_slo[kJoinGrid + 1], s_spre[kJoinGrid + 1];
   const uint64_t moves = st->moves;
-  const uint32_t nh = st->n_holes, ns = st->n_strays;  // nh <= kJoinGrid, ns = nh + 1
-  if (moves == 0) return;
-  for (uint32_t i = threadIdx.x; i < ns; i += blockDim.x) {
-    s_slo[i] = stray_lo[i];
-    s_spre[i] = stray_pre[i];
-    if (i < nh) {
-      s_hlo[i] = hole_lo[i];
-      s_hpre[i] = hole_pre[i];
-    }
-  }
-  __syncthreads();
+  const uint32_t nh = st->n_holes, ns = st->n_strays;
   for (uint64_t k = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; k < moves;
        k += uint64_t(gridDim.x) * blockDim.x) {
-    const uint32_t si = upper_index(s_spre, ns, k);
-    const uint32_t hi = upper_index(s_hpre, nh, k);
-    const uint64_t src = s_slo[si] + (k - s_spre[si]);
-    const uint64_t dst = s_hlo[hi] + (k - s_hpre[hi]);
+    const uint32_t si = upper_index(stray_pre, ns, k);
+    const uint32_t hi = upper_index(hole_pre, nh, k);
+    const uint64_t src = stray_lo[si] + (k - stray_pre[si]);
+    const uint64_t dst = hole_lo[hi] + (k - hole_pre[hi]);
     const uint2 v = src < uint64_t(capacity) ? out[src] : spill[src - uint64_t(capacity)];
     out[dst] = v;
   }
