@@ -505,6 +505,7 @@ struct Carry {
   const uint32_t* start;  // [F] first position of the range per digit
   vec_t* out;             // the pass's output table
   vec_t* sink;            // this lane's sink slot
+  uint32_t n_out;         // its rows (a carried position past them is never stored)
 };
 
 // Per workgroup slot and lane, 16 bytes each; only ever written.
@@ -548,7 +549,8 @@ __device__ __forceinline__ void carry_read(int F, const uint32_t* s_cnt, const u
     const uint32_t dd = d < uint32_t(F) ? d : uint32_t(F) - 1;
     const uint32_t cur = s_cur[dd], old = cur - s_cnt[dd], l0 = old & ~(LT - 1), pos = l0 + k;
     const bool done = last || (cur & ~(LT - 1)) > l0;
-    const bool valid = uint32_t(r) * BLOCK + tid < NS && d < uint32_t(F) && done && pos < old && pos >= cy.start[dd];
+    const bool valid = uint32_t(r) * BLOCK + tid < NS && d < uint32_t(F) && done && pos < old && pos >= cy.start[dd] &&
+                       pos < cy.n_out;
     cr.v[r] = cy.line[slot];
     cr.pos[r] = valid ? pos : 0xFFFFFFFFu;
   }
@@ -560,6 +562,19 @@ __device__ __forceinline__ void carry_write(const Carry<KB>& cy, const CarryRegs
 #pragma unroll
   for (int r = 0; r < CarryRegs<KB, BLOCK, CF>::NR; ++r)
     *(cr.pos[r] != 0xFFFFFFFFu ? cy.out + cr.pos[r] : cy.sink) = cr.v[r];
+}
+
+// Inclusive prefix sum over the 64 lanes of a wave with DPP moves: row shifts
+// by 1, 2, 4 and 8 scan each row of 16 lanes, then row broadcasts of lanes 15
+// and 31 carry the row totals into the later rows (lanes with no source add 0).
+__device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t x) {
+  x += uint32_t(__builtin_amdgcn_update_dpp(0, int(x), 0x111, 0xf, 0xf, false));  // row_shr:1
+  x += uint32_t(__builtin_amdgcn_update_dpp(0, int(x), 0x112, 0xf, 0xf, false));  // row_shr:2
+  x += uint32_t(__builtin_amdgcn_update_dpp(0, int(x), 0x114, 0xf, 0xf, false));  // row_shr:4
+  x += uint32_t(__builtin_amdgcn_update_dpp(0, int(x), 0x118, 0xf, 0xf, false));  // row_shr:8
+  x += uint32_t(__builtin_amdgcn_update_dpp(0, int(x), 0x142, 0xa, 0xf, false));  // row_bcast:15 -> rows 1, 3
+  x += uint32_t(__builtin_amdgcn_update_dpp(0, int(x), 0x143, 0xc, 0xf, false));  // row_bcast:31 -> rows 2, 3
+  return x;
 }
 
 // After rank_tile: the first wave scans the F digit counts (F / 64 per lane,
@@ -578,24 +593,40 @@ __device__ __forceinline__ void scan_stage_tile(const typename ScatterTup<KB>::t
   if (tid < kWave) {
     const int DPL = F > kWave ? F / kWave : 1;  // digits per lane
     const int d0 = tid * DPL;
-    uint32_t local = 0;
-    for (int k = 0; k < DPL; ++k)
-      if (d0 + k < F) local += s_cnt[d0 + k];
-    uint32_t incl = local;
+    if constexpr (CF > 0) {
+      // at most CF / 64 digits per lane: unrolled, so the LDS reads issue together
+      constexpr int DMAX = CF > kWave ? CF / kWave : 1;
+      uint32_t c[DMAX], cur[DMAX], local = 0;
 #pragma unroll
-    for (int dlt = 1; dlt < kWave; dlt <<= 1) {
-      const uint32_t y = __shfl_up(incl, dlt, kWave);
-      if (tid >= dlt) incl += y;
-    }
-    uint32_t run = incl - local;
-    for (int k = 0; k < DPL; ++k) {
-      const int d = d0 + k;
-      if (d < F) {
-        const uint32_t c = s_cnt[d];
-        s_base[d] = run;
-        s_delta[d] = s_cur[d] - run;
-        s_cur[d] += c;
-        run += c;
+      for (int k = 0; k < DMAX; ++k) {
+        const bool ok = k < DPL && d0 + k < F;
+        c[k] = ok ? s_cnt[d0 + k] : 0u;
+        cur[k] = ok ? s_cur[d0 + k] : 0u;
+        local += c[k];
+      }
+      uint32_t run = wave_inclusive_scan(local) - local;
+#pragma unroll
+      for (int k = 0; k < DMAX; ++k)
+        if (k < DPL && d0 + k < F) {
+          s_base[d0 + k] = run;
+          s_delta[d0 + k] = cur[k] - run;
+          s_cur[d0 + k] = cur[k] + c[k];
+          run += c[k];
+        }
+    } else {
+      uint32_t local = 0;
+      for (int k = 0; k < DPL; ++k)
+        if (d0 + k < F) local += s_cnt[d0 + k];
+      uint32_t run = wave_inclusive_scan(local) - local;
+      for (int k = 0; k < DPL; ++k) {
+        const int d = d0 + k;
+        if (d < F) {
+          const uint32_t c = s_cnt[d];
+          s_base[d] = run;
+          s_delta[d] = s_cur[d] - run;
+          s_cur[d] += c;
+          run += c;
+        }
       }
     }
   }
@@ -717,7 +748,7 @@ __device__ __forceinline__ void pass1_body(const PartTable& T, const PartParams&
   uint32_t* s_delta = s_cur + F;
   uint32_t* s_start = s_delta + F;
   tup_t* out = static_cast<tup_t*>(T.buf1);
-  const Carry<KB> cy{reinterpret_cast<tup_t*>(s_start + F), s_start, out, sink_slot<KB>()};
+  const Carry<KB> cy{reinterpret_cast<tup_t*>(s_start + F), s_start, out, sink_slot<KB>(), T.n};
   const int tid = threadIdx.x;
 
   const uint32_t* rel = (pp.passes == 2) ? T.matc + size_t(seg) * F : T.mat + size_t(seg) * F;
@@ -996,7 +1027,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
   };
   const tup_t* in = static_cast<const tup_t*>(T.buf1);
   tup_t* out = static_cast<tup_t*>(T.buf2);
-  const Carry<KB> cy{s_line, s_start, out, sink_slot<KB>()};
+  const Carry<KB> cy{s_line, s_start, out, sink_slot<KB>(), T.n};
 
   auto digits = [&](const tup_t (&tv)[E], uint32_t tile_lo, uint32_t hi, uint32_t (&dg)[E]) {
 #pragma unroll
