@@ -27,7 +27,7 @@ METRIC = "tuples/s & % HBM roofline, 64M⋈256M int32; learned-plan vs plain at 
 def algorithmic_bytes(n_r, n_s, n_out, passes, w=8):
     """Per-phase algorithmic bytes: each pass reads and writes every tuple once,
     the join reads every tuple once and writes 8 bytes per output pair, the
-    histogram reads every key once (reported, not part of the judged total)."""
+    histogram reads every key once (reported, not part of the headline total)."""
     t = n_r + n_s
     return {
         "histogram": t * 4,
