@@ -102,6 +102,56 @@ static std::vector<K> read_keys(const std::string& path, long n) {
   return v;
 }
 
+// TPC-H Q4 text tables as dbgen writes them (one row per line, fields ended
+// by '|'): the build side is orders -- key o_orderkey (field 0), payload the
+// first character of o_orderpriority (field 5) -- and the probe side lineitem
+// -- key l_orderkey (field 0).  A file with fewer rows than requested is
+// repeated from its start (reference: src/generate_zipf_input.cuh:154-253,
+// main.cu:304-333, which reads fixed paths).
+static bool ends_with(const std::string& s, const std::string& suffix) {
+  return s.size() >= suffix.size() && s.compare(s.size() - suffix.size(), suffix.size(), suffix) == 0;
+}
+
+template <typename K>
+static std::vector<K> read_tbl(const std::string& path, long n, int pay_field, std::vector<int32_t>* pay) {
+  std::ifstream f(path);
+  if (!f) {
+    std::cerr << "Error: cannot open " << path << std::endl;
+    std::exit(1);
+  }
+  std::vector<K> keys;
+  keys.reserve(static_cast<size_t>(n));
+  std::string line;
+  while (long(keys.size()) < n && std::getline(f, line)) {
+    if (line.empty()) continue;
+    size_t start = 0;
+    std::string key_field, pay_text;
+    for (int field = 0; field <= std::max(0, pay_field); ++field) {
+      const size_t bar = line.find('|', start);
+      if (bar == std::string::npos) {
+        std::cerr << "Error: " << path << " line " << keys.size() + 1 << " has fewer than " << field + 1
+                  << " fields" << std::endl;
+        std::exit(1);
+      }
+      if (field == 0) key_field = line.substr(start, bar - start);
+      if (field == pay_field) pay_text = line.substr(start, bar - start);
+      start = bar + 1;
+    }
+    keys.push_back(static_cast<K>(std::strtoll(key_field.c_str(), nullptr, 10)));
+    if (pay) pay->push_back(pay_text.empty() ? 0 : int32_t(static_cast<unsigned char>(pay_text[0])));
+  }
+  if (keys.empty() && n > 0) {
+    std::cerr << "Error: " << path << " holds no rows" << std::endl;
+    std::exit(1);
+  }
+  const size_t rows = keys.size();
+  for (size_t i = rows; i < size_t(n); ++i) {
+    keys.push_back(keys[i % rows]);
+    if (pay) pay->push_back((*pay)[i % rows]);
+  }
+  return keys;
+}
+
 template <typename K>
 static int run(int argc, char** argv, int key_bytes) {
   long build_n = 1 << 20, probe_n = 1 << 20;
@@ -200,7 +250,15 @@ static int run(int argc, char** argv, int key_bytes) {
   HIP_OK(hipMalloc(&d_probe, std::max(1L, probe_n) * sizeof(K)));
   hipStream_t s;
   HIP_OK(hipStreamCreate(&s));
-  if (from_file) {
+  std::vector<int32_t> bpay;  // build payload read from a file (the pairs then carry it)
+  if (from_file && ends_with(bfile, ".tbl") && ends_with(pfile, ".tbl")) {
+    auto hb = read_tbl<K>(bfile, build_n, 5, &bpay);
+    auto hp = read_tbl<K>(pfile, probe_n, 0, nullptr);
+    HIP_OK(hipMemcpy(d_build, hb.data(), build_n * sizeof(K), hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(d_probe, hp.data(), probe_n * sizeof(K), hipMemcpyHostToDevice));
+    HIP_OK(hipMalloc(&d_bval, std::max(1L, build_n) * 4));
+    HIP_OK(hipMemcpy(d_bval, bpay.data(), build_n * 4, hipMemcpyHostToDevice));
+  } else if (from_file) {
     auto hb = read_keys<K>(bfile, build_n), hp = read_keys<K>(pfile, probe_n);
     HIP_OK(hipMemcpy(d_build, hb.data(), build_n * sizeof(K), hipMemcpyHostToDevice));
     HIP_OK(hipMemcpy(d_probe, hp.data(), probe_n * sizeof(K), hipMemcpyHostToDevice));
@@ -245,9 +303,11 @@ static int run(int argc, char** argv, int key_bytes) {
       PJ_OK_OR_DIE(pj_gen_probe_i64(reinterpret_cast<int64_t*>(d_probe), probe_n, build_n, 1234, 5678, sel, s));
   }
   if (with_value) {
-    HIP_OK(hipMalloc(&d_bval, std::max(1L, build_n) * 4));
+    if (!d_bval) {
+      HIP_OK(hipMalloc(&d_bval, std::max(1L, build_n) * 4));
+      PJ_OK_OR_DIE(pj_gen_row_ids(d_bval, build_n, s));
+    }
     HIP_OK(hipMalloc(&d_pval, std::max(1L, probe_n) * 4));
-    PJ_OK_OR_DIE(pj_gen_row_ids(d_bval, build_n, s));
     PJ_OK_OR_DIE(pj_gen_row_ids(d_pval, probe_n, s));
   }
   HIP_OK(hipStreamSynchronize(s));
@@ -288,6 +348,8 @@ static int run(int argc, char** argv, int key_bytes) {
   std::vector<std::pair<int32_t, int32_t>> cpu_pairs;
   double cpu_s = -1;
   if (!skip_cpu) cpu_s = cpu_join<K>(hb, hp, uniq, &cpu_pairs);
+  for (auto& pr : cpu_pairs)
+    if (!bpay.empty()) pr.first = bpay[size_t(pr.first)];  // the GPU pairs carry the build payload
   const int64_t cap = skip_cpu ? std::max(1L, out_size) : std::max<int64_t>(1, int64_t(cpu_pairs.size()));
 
   // ---------------------------------------------------------------- GPU run

@@ -79,3 +79,38 @@ def test_key_files(tmp_path):
     # a file shorter than the requested size is an error, not a short read
     r = _run(["-C", "-F", "-B", str(bf), "-P", str(pf), "-b", str(build.size + 1), "-p", str(probe.size)])
     assert r.returncode == 1 and "fewer than" in r.stderr
+
+
+def _write_tpch(tmp_path, n_orders, rng):
+    """Synthetic dbgen-style orders.tbl / lineitem.tbl: sparse unique order
+    keys, 1-7 lines per order, priorities '1-URGENT' .. '5-LOW'."""
+    prios = ["1-URGENT", "2-HIGH", "3-MEDIUM", "4-NOT SPECIFIED", "5-LOW"]
+    okeys = (np.arange(n_orders, dtype=np.int64) // 8) * 32 + (np.arange(n_orders) % 8) + 1
+    oprio = rng.integers(0, 5, n_orders)
+    with open(tmp_path / "orders.tbl", "w") as f:
+        for k, p in zip(okeys.tolist(), oprio.tolist()):
+            f.write(f"{k}|{k % 1000}|O|1234.56|1996-01-02|{prios[p]}|Clerk#000000951|0|regular deposits|\n")
+    lines = rng.integers(1, 8, n_orders)
+    lkeys = np.repeat(okeys, lines)
+    with open(tmp_path / "lineitem.tbl", "w") as f:
+        for i, k in enumerate(lkeys.tolist()):
+            f.write(f"{k}|{i % 977}|{i % 31}|{i % 7 + 1}|17|21168.23|0.04|0.02|N|O|1996-03-13|1996-02-12|"
+                    f"1996-03-22|DELIVER IN PERSON|TRUCK|egular courts above the|\n")
+    return okeys, oprio, lkeys
+
+
+@pytest.mark.gpu
+def test_tpch_q4_tables(tmp_path):
+    """TPC-H Q4 input (-F with .tbl files): build = orders' o_orderkey with the
+    priority's first character as payload, probe = lineitem's l_orderkey; a
+    probe request past the file's rows repeats the file."""
+    rng = np.random.default_rng(11)
+    okeys, oprio, lkeys = _write_tpch(tmp_path, 20000, rng)
+    n_probe = len(lkeys) + 12345
+    f = _csv(["-F", "-B", str(tmp_path / "orders.tbl"), "-P", str(tmp_path / "lineitem.tbl"), "-b", str(len(okeys)),
+              "-p", str(n_probe), "-R", "-n", "16", "-N", "16"])
+    assert f[-1] == "0", f
+    assert int(f[11]) == n_probe  # every lineitem row (and repeat) has exactly one order
+    r = _run(["-C", "-F", "-B", str(tmp_path / "lineitem.tbl"), "-P", str(tmp_path / "missing.tbl"), "-b", "10",
+              "-p", "10"])
+    assert r.returncode == 1 and "cannot open" in r.stderr
