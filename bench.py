@@ -42,7 +42,7 @@ def run_join_steps(pj, torch, bk, pk, plan, steps, warmup):
     """warmup + timed steps of the async join; returns (ms/step, phase means, count)."""
     n_cap = pk.numel() + 1024
     out = torch.empty((n_cap, 2), dtype=torch.int32, device="cuda")
-    ws_bytes = pj.workspace_size(bk.numel(), pk.numel(), 4, plan)
+    ws_bytes = pj.workspace_size(bk.numel(), pk.numel(), bk.element_size(), plan)
     ws = torch.empty(ws_bytes, dtype=torch.uint8, device="cuda")
     for _ in range(warmup):
         d_count = pj.partitioned_join_async(bk, pk, out, plan=plan, workspace=ws)

@@ -472,17 +472,20 @@ __device__ __forceinline__ void rank_tile(const uint32_t (&dg)[E], uint32_t* s_c
 }
 
 // The scatter kernels hold and move tuples as vectors of the tuple's bytes
-// (uint2: key, payload; uint4: key low, key high, payload, 0): a struct value
-// merged over branches or copied between register sets is kept in scratch
-// memory, whose accesses join the in-order memory counter.  The key's low
+// (uint2: key, payload; four words: key low, key high, payload, 0): a struct
+// value merged over branches or copied between register sets is kept in
+// scratch memory, whose accesses join the in-order memory counter.  16-byte
+// tuples are a native 4-word vector rather than HIP's uint4, a union whose
+// copies go through 12-byte memory temporaries in scratch.  The key's low
 // 32 bits are word x in both.
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 template <int KB>
 struct ScatterTup {
-  using type = std::conditional_t<KB == 4, uint2, uint4>;
+  using type = std::conditional_t<KB == 4, uint2, u32x4>;
   static_assert(sizeof(type) == sizeof(typename KeyT<KB>::tup_t), "tuple and vector sizes differ");
   __device__ __forceinline__ static type make(typename KeyT<KB>::key_t k, int32_t p) {
     if constexpr (KB == 4) return make_uint2(uint32_t(k), uint32_t(p));
-    else return make_uint4(uint32_t(k), uint32_t(uint64_t(k) >> 32), uint32_t(p), 0u);
+    else return u32x4{uint32_t(k), uint32_t(uint64_t(k) >> 32), uint32_t(p), 0u};
   }
 };
 
@@ -701,7 +704,9 @@ __host__ __device__ constexpr size_t scatter_lds_bytes(int F) {
 // per thread, a multiple of 4 in pass 1, whose loads are 16 bytes wide):
 //   4-byte keys  F <= 64: 16K + carry; F <= 256: 12K (pass 1) or 14K (pass 2)
 //                + carry; 512: 8K + carry; 1024: 16K; more: 8K;
-//   8-byte keys  (16-byte tuples) F <= 1024: 8K; more: 4K, without a carry.
+//   8-byte keys  (16-byte tuples) F <= 64: 8K + carry (pass 2), F <= 128: 8K +
+//                carry (pass 1); F <= 256: 4K (pass 1) or 7K (pass 2) + carry;
+//                512: 4K + carry; 1024: 8K; more: 4K.
 struct ScatterCfg {
   int tile, cf;
 };
@@ -713,10 +718,11 @@ inline ScatterCfg scatter_cfg(int key_bytes, int F, bool pass2) {
     if (F == 1024) return {16384, 0};
     return {8192, 0};
   }
-  // 16-byte tuples: the carry variants keep their register arrays in scratch
-  // memory (measured slower on 64-bit-key joins), so they scatter without one
-  (void)pass2;
-  return {F <= 1024 ? 8192 : 4096, 0};
+  if (F <= (pass2 ? 64 : 128)) return {8192, pass2 ? 64 : 128};
+  if (F <= 256) return {pass2 ? 7168 : 4096, pass2 ? 256 : 512};
+  if (F == 512) return {4096, 512};
+  if (F == 1024) return {8192, 0};
+  return {4096, 0};
 }
 
 // ---------------------------------------------------------------------------
@@ -920,6 +926,8 @@ static hipError_t pass1_launch(const Layout& L, hipStream_t s) {
     PJ_P1(16384, 0);
     PJ_P1(8192, 0);
   } else {
+    PJ_P1(8192, 128);
+    PJ_P1(4096, 512);
     PJ_P1(8192, 0);
     PJ_P1(4096, 0);
   }
@@ -1151,6 +1159,9 @@ hipError_t launch_pass2(const Layout& L, int key_bytes, hipStream_t s) {
     PJ_PASS2(4, 16384, 0)
     PJ_PASS2(4, 8192, 0)
   } else {
+    PJ_PASS2(8, 8192, 64)
+    PJ_PASS2(8, 7168, 256)
+    PJ_PASS2(8, 4096, 512)
     PJ_PASS2(8, 8192, 0)
     PJ_PASS2(8, 4096, 0)
   }
