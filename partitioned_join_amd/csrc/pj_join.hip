@@ -390,12 +390,14 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
             key_t kk[BS];
             bucket_keys(*reinterpret_cast<const uint4*>(&s_keys[slot[k]]), kk);
             const key_t key = KTr::key(ptr[k]);
-            uint32_t hit = BS, empty = 0;
+            // A bucket's filled slots are a prefix of it: keys start at a
+            // bucket's first slot and only move on past occupied slots, and
+            // the table (a whole number of buckets) wraps to bucket 0.  So
+            // the bucket has an empty slot iff its last slot is empty.
+            const bool empty = kk[BS - 1] == sentinel;
+            uint32_t hit = BS;
 #pragma unroll
-            for (int i = BS - 1; i >= 0; --i) {
-              hit = kk[i] == key ? uint32_t(i) : hit;
-              empty |= kk[i] == sentinel ? 1u : 0u;
-            }
+            for (int i = BS - 1; i >= 0; --i) hit = kk[i] == key ? uint32_t(i) : hit;
             if (hit < BS) {
               found |= 1u << k;
               walk &= ~(1u << k);
