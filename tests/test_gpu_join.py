@@ -153,6 +153,23 @@ def test_int64_keys(pj, oracle, torch):
     check_join(pj, oracle, torch, dup, dup[::-1].copy(), uniq=False)
 
 
+@pytest.mark.parametrize("plan", [
+    {"passes": 2, "bits1": 6, "bits2": 6},   # pass 1: 8K tile + 128-digit carry; pass 2: 8K + 64-digit carry
+    {"passes": 2, "bits1": 7, "bits2": 7},   # pass 2: 7K tile + 256-digit carry
+    {"passes": 2, "bits1": 8, "bits2": 8},   # pass 1: 4K tile + 512-digit carry
+    {"passes": 1, "bits1": 9},               # one pass, fan-out 512: 4K + 512-digit carry
+    {"passes": 2, "bits1": 5, "bits2": 9},   # pass 2 fan-out 512: 4K + 512-digit carry
+    {"passes": 2, "bits1": 4, "bits2": 10},  # pass 2 fan-out 1024: 8K, no carry
+])
+def test_int64_scatter_configurations(pj, oracle, torch, plan):
+    """Every 16-byte-tuple scatter configuration (tile and line carry chosen
+    by the fan-out of each pass) on 64-bit keys, against the oracle."""
+    G = pj.generators
+    bk = G.build_i64(300000, 0x65)
+    pk = G.probe_i64(900000, 300000, 0x65, 0x66, 0.6)
+    check_join(pj, oracle, torch, bk.cpu().numpy(), pk.cpu().numpy(), plan=plan)
+
+
 def test_capacity_error_reports_count(pj, torch):
     from partitioned_join_amd import PjError
     bk = torch.arange(1000, dtype=torch.int32, device="cuda")
