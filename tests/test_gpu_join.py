@@ -170,6 +170,20 @@ def test_int64_scatter_configurations(pj, oracle, torch, plan):
     check_join(pj, oracle, torch, bk.cpu().numpy(), pk.cpu().numpy(), plan=plan)
 
 
+@pytest.mark.parametrize("plan", [{"passes": 2, "bits1": 7, "bits2": 7}, {"passes": 1, "bits1": 9}])
+def test_int64_payload_columns_and_duplicates(pj, oracle, torch, plan):
+    """64-bit keys with duplicate build keys and int32 payload columns on both
+    sides (the payload variants of the 16-byte-tuple scatters)."""
+    rng = np.random.default_rng(24)
+    bk = rng.integers(-2**62, 2**62, 150000).astype(np.int64)
+    bk[1::2] = bk[0::2]  # every key twice
+    bk[::500] = bk[3]    # and one key 300 times
+    pk = np.concatenate([rng.choice(bk, 400000), rng.integers(-2**62, 2**62, 100000)]).astype(np.int64)
+    bp = rng.integers(-2**31, 2**31 - 1, len(bk)).astype(np.int32)
+    pp = rng.integers(-2**31, 2**31 - 1, len(pk)).astype(np.int32)
+    check_join(pj, oracle, torch, bk, pk, bp, pp, plan=plan, uniq=False)
+
+
 def test_capacity_error_reports_count(pj, torch):
     from partitioned_join_amd import PjError
     bk = torch.arange(1000, dtype=torch.int32, device="cuda")
