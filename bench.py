@@ -234,9 +234,11 @@ def main():
         from partitioned_join_amd import planner
         w = planner.load_weights()
         if w is not None:
+            planner.plan_learned(zb, zp, w)  # first call: loads the planner's kernels
             t0 = time.perf_counter()
-            lplan, fallback = planner.plan_learned(zb, zp, w)
-            plan_ms = (time.perf_counter() - t0) * 1000.0
+            for _ in range(3):
+                lplan, fallback = planner.plan_learned(zb, zp, w)
+            plan_ms = (time.perf_counter() - t0) * 1000.0 / 3
             lms, lph, lcount = run_join_steps(pj, torch, zb, zp, lplan, max(3, args.steps // 2), 1)
             assert lcount == zcount, (lcount, zcount)
             rec.update({"learned_tuples_per_s": (zb.numel() + zp.numel()) / (lms / 1000.0), "learned_ms": lms,

@@ -6,7 +6,8 @@
 //                     not which bins it falls in), six size/skew scalars, and
 //                     the 64 most frequent sampled keys as heavy candidates.
 //   k_count_cand      exact occurrence counts of the candidates in the whole
-//                     probe column (LDS hash of the candidates, one pass).
+//                     probe column (LDS hash of the candidates, one pass;
+//                     16-byte vector loads when the column is aligned).
 //   k_gemm, ...       a 1030 -> 256 -> 256 -> 81 FP32 MLP: forward, backward
 //                     and Adam.  The products are tiny (batch <= a few hundred)
 //                     so one LDS-tiled FP32 GEMM with strided operands serves
@@ -167,6 +168,74 @@ __global__ __launch_bounds__(256) void k_count_cand(const void* keys, int64_t n,
     if (s_cnt[c]) atomicAdd(&counts[c], (unsigned long long)s_cnt[c]);
 }
 
+// The same count over a 16-byte-aligned column, read as 16-byte vectors with
+// two loads in flight per thread; the n % (16 / key bytes) last keys are read
+// one by one by the first workgroup.  The table has 16 slots per candidate,
+// so a key that is no candidate (nearly all of them) mostly ends its walk at
+// the first slot it reads.
+template <int KB>
+__global__ __launch_bounds__(256) void k_count_cand_vec(const void* keys, int64_t n, const uint64_t* cand_keys,
+                                                        const uint32_t* cand_sample_counts,
+                                                        unsigned long long* counts) {
+  using key_t = typename KeyT<KB>::key_t;
+  constexpr int SLOTS = 16 * kPlanCand;
+  constexpr int PER = 16 / KB;  // keys per vector
+  __shared__ uint64_t s_key[SLOTS];
+  __shared__ int32_t s_idx[SLOTS];
+  __shared__ uint32_t s_cnt[kPlanCand];
+  const int tid = threadIdx.x;
+  for (int i = tid; i < SLOTS; i += 256) s_idx[i] = -1;
+  for (int i = tid; i < kPlanCand; i += 256) s_cnt[i] = 0;
+  __syncthreads();
+  if (tid == 0) {
+    for (int c = 0; c < kPlanCand; ++c) {
+      if (cand_sample_counts[c] == 0) continue;
+      const key_t key = key_t(cand_keys[c]);
+      uint32_t s = murmur3(key) & uint32_t(SLOTS - 1);
+      while (s_idx[s] >= 0) s = (s + 1) & uint32_t(SLOTS - 1);
+      s_key[s] = uint64_t(key);
+      s_idx[s] = c;
+    }
+  }
+  __syncthreads();
+  auto lookup = [&](key_t key) {
+    uint64_t v = uint64_t(key);
+    if (KB == 4) v &= 0xffffffffull;
+    uint32_t s = murmur3(key) & uint32_t(SLOTS - 1);
+    while (s_idx[s] >= 0) {  // at most 64 of the slots are used
+      if (s_key[s] == v) {
+        atomicAdd(&s_cnt[s_idx[s]], 1u);
+        break;
+      }
+      s = (s + 1) & uint32_t(SLOTS - 1);
+    }
+  };
+  auto lookup_vec = [&](const uint4 q) {
+    if constexpr (KB == 4) {
+      lookup(q.x);
+      lookup(q.y);
+      lookup(q.z);
+      lookup(q.w);
+    } else {
+      lookup(uint64_t(q.x) | (uint64_t(q.y) << 32));
+      lookup(uint64_t(q.z) | (uint64_t(q.w) << 32));
+    }
+  };
+  const uint4* k4 = static_cast<const uint4*>(keys);
+  const int64_t nv = n / PER, stride = int64_t(gridDim.x) * 256;
+  int64_t i = blockIdx.x * int64_t(256) + tid;
+  for (; i + stride < nv; i += 2 * stride) {
+    const uint4 a = k4[i], b = k4[i + stride];
+    lookup_vec(a);
+    lookup_vec(b);
+  }
+  if (i < nv) lookup_vec(k4[i]);
+  if (blockIdx.x == 0 && int64_t(tid) < n - nv * PER) lookup(static_cast<const key_t*>(keys)[nv * PER + tid]);
+  __syncthreads();
+  for (int c = tid; c < kPlanCand; c += 256)
+    if (s_cnt[c]) atomicAdd(&counts[c], (unsigned long long)s_cnt[c]);
+}
+
 hipError_t planner_sketch(const void* keys, int64_t n, int key_bytes, int64_t n_build, float* feat,
                           uint64_t* cand_keys, uint32_t* cand_counts, hipStream_t s) {
   if (key_bytes == 4)
@@ -181,6 +250,18 @@ hipError_t planner_count_candidates(const void* keys, int64_t n, int key_bytes, 
                                     hipStream_t s) {
   hipError_t e = launch_fill_u64(reinterpret_cast<uint64_t*>(counts), 0, kPlanCand, s);
   if (e != hipSuccess || n == 0) return e;
+  if (reinterpret_cast<uintptr_t>(keys) % 16 == 0) {
+    int64_t g = (n / (16 / key_bytes) + 511) / 512;
+    if (g < 1) g = 1;
+    if (g > 4096) g = 4096;
+    if (key_bytes == 4)
+      hipLaunchKernelGGL(k_count_cand_vec<4>, dim3(int(g)), dim3(256), 0, s, keys, n, cand_keys, cand_sample_counts,
+                         counts);
+    else
+      hipLaunchKernelGGL(k_count_cand_vec<8>, dim3(int(g)), dim3(256), 0, s, keys, n, cand_keys, cand_sample_counts,
+                         counts);
+    return hipGetLastError();
+  }
   int64_t g = (n + 255) / 256;
   if (g > 4096) g = 4096;
   if (key_bytes == 4)

@@ -192,6 +192,11 @@ def test_sketch_and_candidate_counts(pj, oracle, torch, dtype):
     for j in range(64):
         want = int((pu == ek[j]).sum()) if ec[j] > 0 else 0
         assert exact[j] == want, j
+    # a column that starts one key in is not 16-byte aligned: the per-key kernel
+    shifted = planner.candidate_counts(d(probe)[1:], cand, cnt).cpu().numpy()
+    for j in range(64):
+        want = int((pu[1:] == ek[j]).sum()) if ec[j] > 0 else 0
+        assert shifted[j] == want, j
     # empty probe: no candidates, zero features
     feat0, _, cnt0 = planner.sketch(d(build), d(probe[:0]))
     assert float(feat0[:1024].abs().sum()) == 0.0 and int(cnt0.sum()) == 0
