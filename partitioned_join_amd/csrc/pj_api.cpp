@@ -870,13 +870,16 @@ pj_status pj_plan_learned(const pj_table* build, const pj_table* probe, const fl
   PJ_HIP(planner_sketch(probe->keys, probe->n, probe->key_bytes, build->n, feat, cand, cand_n, stream));
   PJ_HIP(planner_forward(d_weights, feat, 1, logits, scratch, stream));
   PJ_HIP(planner_count_candidates(probe->keys, probe->n, probe->key_bytes, cand, cand_n, exact, stream));
-  float h_logits[kPlanOut];
-  uint32_t h_cand_n[kPlanCand];
-  uint64_t h_exact[kPlanCand];
-  PJ_HIP(hipMemcpyAsync(h_logits, logits, sizeof(h_logits), hipMemcpyDeviceToHost, stream));
-  PJ_HIP(hipMemcpyAsync(h_cand_n, cand_n, sizeof(h_cand_n), hipMemcpyDeviceToHost, stream));
-  PJ_HIP(hipMemcpyAsync(h_exact, exact, sizeof(h_exact), hipMemcpyDeviceToHost, stream));
+  // cand_n, exact and logits lie in that order in the workspace: one copy
+  const char* lo = reinterpret_cast<const char*>(cand_n);
+  const size_t span = size_t(reinterpret_cast<const char*>(logits + kPlanOut) - lo);
+  alignas(16) char h_span[((kPlanCand * 4 + 255) & ~255) + ((kPlanCand * 8 + 255) & ~255) + kPlanOut * sizeof(float)];
+  if (span != sizeof(h_span)) return fail(PJ_ERR_UNSUPPORTED, "planner workspace layout changed");
+  PJ_HIP(hipMemcpyAsync(h_span, lo, span, hipMemcpyDeviceToHost, stream));
   PJ_HIP(hipStreamSynchronize(stream));
+  const float* h_logits = reinterpret_cast<const float*>(h_span + (reinterpret_cast<const char*>(logits) - lo));
+  const uint32_t* h_cand_n = reinterpret_cast<const uint32_t*>(h_span);
+  const uint64_t* h_exact = reinterpret_cast<const uint64_t*>(h_span + (reinterpret_cast<const char*>(exact) - lo));
   return pj_plan_decode(h_logits, h_cand_n, h_exact, build, probe, plan, fallback);
 }
 

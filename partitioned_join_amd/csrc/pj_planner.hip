@@ -103,10 +103,9 @@ __global__ __launch_bounds__(1024) void k_sketch(const void* keys, int64_t n, in
   if (tid == 0) top = float(uint32_t(kPlanSample) - uint32_t(s_sel[0] >> 12));
   __syncthreads();
   // Histogram bins sorted by count, descending (reuse s_keys).
-  for (int b = tid; b < kPlanSample; b += 1024)
-    s_keys[b] = b < kPlanBins ? ~uint64_t(s_hist[b]) : ~uint64_t(0) ;
+  for (int b = tid; b < kPlanBins; b += 1024) s_keys[b] = ~uint64_t(s_hist[b]);
   __syncthreads();
-  bitonic_sort_u64<kPlanSample>(s_keys);
+  bitonic_sort_u64<kPlanBins>(s_keys);
   for (int b = tid; b < kPlanBins; b += 1024) feat[b] = float(~s_keys[b]) / float(kPlanSample);
   if (tid == 0) {
     float tot = 0.f;
