@@ -152,7 +152,10 @@ static void plain_plan(int64_t nR, int key_bytes, pj_plan* plan) {
   plan->unique_build_keys = 0;
   plan->probe_slice = 0;
   plan->source = 0;
-  if (B <= 11) {
+  // One pass up to 1024 partitions; from 2048 on, two passes beat one scatter
+  // over that many runs (int64 2M x 8M: 0.50 ms in one pass of 11 bits, 0.42 ms
+  // in 6 + 5; at 10 bits one pass stays ahead, 0.22 against 0.27 ms for 1M x 4M).
+  if (B <= 10) {
     plan->passes = 1;
     plan->bits1 = B;
     plan->bits2 = 0;
