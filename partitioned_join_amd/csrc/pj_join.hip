@@ -621,20 +621,52 @@ __device__ __forceinline__ uint32_t upper_index(const uint64_t* pre, uint32_t n,
   return lo;
 }
 
+// Workgroup i moves stray run i: moves k in [stray_pre[i], stray_pre[i+1])
+// (below `moves`) go from stray_lo[i] + (k - stray_pre[i]) to the hole holding
+// move k.  One search finds the run's first hole; the workgroup then walks the
+// holes in order, copying each overlap with contiguous reads and writes.
+// Every index is uniform in the workgroup; each thread reads four pairs
+// before it writes them, so a run's copy keeps many loads in flight, and a
+// long run is split between several workgroups.
 __global__ __launch_bounds__(256) void k_defrag_move(const DefragState* st, const uint64_t* hole_lo,
                                                      const uint64_t* hole_pre, const uint64_t* stray_lo,
                                                      const uint64_t* stray_pre, uint2* out, int64_t capacity,
                                                      const uint2* spill) {
   const uint64_t moves = st->moves;
   const uint32_t nh = st->n_holes, ns = st->n_strays;
-  for (uint64_t k = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; k < moves;
-       k += uint64_t(gridDim.x) * blockDim.x) {
-    const uint32_t si = upper_index(stray_pre, ns, k);
-    const uint32_t hi = upper_index(hole_pre, nh, k);
-    const uint64_t src = stray_lo[si] + (k - stray_pre[si]);
-    const uint64_t dst = hole_lo[hi] + (k - hole_pre[hi]);
-    const uint2 v = src < uint64_t(capacity) ? out[src] : spill[src - uint64_t(capacity)];
-    out[dst] = v;
+  const uint32_t i = blockIdx.x;
+  if (i >= ns || moves == 0) return;
+  const uint64_t r0 = stray_pre[i], r1 = min(stray_pre[i + 1], moves);
+  if (r0 >= r1) return;
+  const uint64_t s0 = stray_lo[i] - r0;  // source of move k: s0 + k
+  // blockIdx.y takes one of gridDim.y equal slices of the run
+  const uint64_t sl = (r1 - r0 + gridDim.y - 1) / gridDim.y;
+  uint64_t k = r0 + sl * blockIdx.y;
+  const uint64_t kend = min(r1, k + sl);
+  if (k >= kend) return;
+  // k < moves <= hole_pre[nh], so the hole of every move here is below nh
+  uint32_t h = upper_index(hole_pre, nh, k);
+  while (k < kend) {
+    const uint64_t hend = hole_pre[h + 1];
+    const uint64_t d0 = hole_lo[h] - hole_pre[h];  // destination of move k: d0 + k
+    const uint64_t e = min(kend, hend);
+    constexpr uint32_t U = 4;
+    for (uint64_t base = k; base < e; base += uint64_t(U) * blockDim.x) {
+      uint2 v[U];
+#pragma unroll
+      for (uint32_t u = 0; u < U; ++u) {
+        const uint64_t kk = base + threadIdx.x + uint64_t(u) * blockDim.x;
+        const uint64_t src = s0 + kk;
+        if (kk < e) v[u] = src < uint64_t(capacity) ? out[src] : spill[src - uint64_t(capacity)];
+      }
+#pragma unroll
+      for (uint32_t u = 0; u < U; ++u) {
+        const uint64_t kk = base + threadIdx.x + uint64_t(u) * blockDim.x;
+        if (kk < e) out[d0 + kk] = v[u];
+      }
+    }
+    k = e;
+    ++h;
   }
 }
 
@@ -700,7 +732,7 @@ hipError_t launch_defrag(const Layout& L, pj_pair* out, int64_t capacity, int64_
                      L.dstate, L.hole_lo, L.hole_pre, L.stray_lo, L.stray_pre);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_defrag_move, dim3(512), dim3(256), 0, s, L.dstate, L.hole_lo, L.hole_pre, L.stray_lo,
+  hipLaunchKernelGGL(k_defrag_move, dim3(kJoinGrid + 1, 4), dim3(256), 0, s, L.dstate, L.hole_lo, L.hole_pre, L.stray_lo,
                      L.stray_pre, reinterpret_cast<uint2*>(out), capacity, reinterpret_cast<const uint2*>(L.spill));
   return hipGetLastError();
 }
