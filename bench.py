@@ -5,9 +5,12 @@ LDS hash join, output compaction) with inputs resident in HBM.
 
 Prints one JSON line: whole-job tuples/s, the roofline of the dominant kernel
 (algorithmic bytes / its hipEvent-timed duration vs 8 TB/s), per-phase times,
-the plain plan on the Zipf(1.0) skewed join, and a CPU baseline: the
-reference's own inner_join_cpu (compiled from its sources into oracle/_ref)
-timed on this host on a bounded sample.
+the plain and learned plans on the Zipf(1.0) skewed join, the batched-joins
+record, and a CPU baseline: the oracle's restatement of the reference's
+inner_join_cpu (std::unordered_map, serial build, OpenMP probe; the
+reference's own CPU code needs the CUDA toolkit and is not built here) timed
+on this host's cores on the full 64M x 256M workload, the 1M x 4M one and the
+16M x 256M Zipf(1.0) one.
 """
 import argparse
 import json
@@ -91,7 +94,7 @@ def measured_traffic(kernel):
 def batched_joins(pj, torch, count=512):
     """512 independent int64-key joins of uneven sizes replayed as one HIP
     graph: the untuned default (8 chains, plain plans) and the autotuned
-    configuration over chains x fan-out with learned per-join plans."""
+    configuration over chains x fan-out, with plain per-join plans."""
     from partitioned_join_amd import batch
     builds, probes, specs = batch.make_batch_workload(count)
     tuples = sum(s["n_build"] + s["n_probe"] for s in specs)
@@ -100,9 +103,8 @@ def batched_joins(pj, torch, count=512):
     ms_default = bj.time(3)
     counts_default = bj.counts()
     bj.close()
-    learned = batch.batch_plans(builds, probes, "learned")
     t0 = time.perf_counter()
-    best, cfg, timings = batch.autotune(builds, probes, learned, chain_options=(8, 16, 32), bit_offsets=(-1, 0, 1),
+    best, cfg, timings = batch.autotune(builds, probes, plain, chain_options=(8, 16, 32), bit_offsets=(-1, 0, 1),
                                         reps=3)
     tune_s = time.perf_counter() - t0
     ms_best = best.time(5)
@@ -114,7 +116,7 @@ def batched_joins(pj, torch, count=512):
             "tuples": tuples, "pairs": int(counts_best.sum()),
             "default_ms": ms_default, "default_tuples_per_s": tuples / (ms_default / 1000.0),
             "tuned_ms": ms_best, "tuned_tuples_per_s": tuples / (ms_best / 1000.0),
-            "tuned_config": {"chains": cfg[0], "fanout_offset_bits": cfg[1], "plans": "learned with plain fallback"},
+            "tuned_config": {"chains": cfg[0], "fanout_offset_bits": cfg[1], "plans": "plain"},
             "autotune_seconds": tune_s,
             "timings_ms": {f"chains={c},fanout{d:+d}": v for (c, d), v in timings.items()}}
 
@@ -134,30 +136,43 @@ def batched_joins_child(timeout_s=600):
     return json.loads(lines[-1])
 
 
-def cpu_baseline(scale_shift=2):
-    """Reference inner_join_cpu (oracle/_ref) on the same distribution at
-    1/2^scale_shift size; falls back to the oracle restatement if the
-    reference library is not present."""
-    import numpy as np
+def cpu_info():
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for l in f:
+                if l.startswith("model name"):
+                    model = l.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "model": model}
+
+
+def cpu_join(bk, pk, threads):
+    """The oracle's inner_join_cpu (unique build keys) on host key arrays:
+    (tuples/s, pairs, seconds in the omp_get_wtime window of build + probe,
+    src/inner_join_cpu.hpp:41-78)."""
     from oracle import oracle as O
-    nb, npr = (1 << 26) >> scale_shift, (1 << 28) >> scale_shift
-    bk = O.gen_build_unique_i32(nb, nb, 0xC2B)
-    pk = O.gen_probe_i32(npr, nb, nb, 0xC2B, 0xC2F, 0.5, nb, nb)
-    cap = int((pk < nb).sum())
-    threads = os.cpu_count() or 1
-    res = O.ref_inner_join_cpu(bk, pk, uniq_keys=True, num_threads=threads, capacity=cap)
-    kind = "reference"
-    if res is None:
-        _, n, secs = O.inner_join_cpu(bk, pk, uniq_keys=True, num_threads=threads, capacity=cap,
-                                      return_seconds=True)
-        kind = "port"
-    else:
-        _, n, secs = res
-    assert n == cap, (n, cap)
-    return {"value": (nb + npr) / secs, "unit": "tuples/s", "cores": threads, "kind": kind,
-            "sample": f"{nb >> 20}M unique x {npr >> 20}M probe int32 (1/{1 << scale_shift} of the 64M x 256M "
-                      f"workload, same generator), build+probe wall time {secs:.3f} s "
-                      "(omp_get_wtime window of inner_join_cpu)"}
+    cap = len(pk)  # unique build keys: at most one pair per probe row
+    _, n, secs = O.inner_join_cpu(bk, pk, uniq_keys=True, num_threads=threads, capacity=cap, return_seconds=True)
+    return (len(bk) + len(pk)) / secs, n, secs
+
+
+def cpu_baseline(c2_host, c1_host, c3_host, threads):
+    """CPU join on the headline workload (64M x 256M, the same keys the GPU
+    joined), plus the 1M x 4M and 16M x 256M Zipf(1.0) workloads."""
+    v, n, secs = cpu_join(*c2_host, threads)
+    rec = {"value": v, "unit": "tuples/s", "cores": threads, "kind": "port",
+           "sample": f"full 64M x 256M workload (the bench's own keys, copied to the host): {n} pairs, "
+                     f"build+probe {secs:.2f} s in the omp_get_wtime window of inner_join_cpu",
+           "host": cpu_info()}
+    v1, n1, s1 = cpu_join(*c1_host, threads)
+    rec["config1_1Mx4M"] = {"tuples_per_s": v1, "pairs": n1, "seconds": s1}
+    if c3_host is not None:
+        v3, n3, s3 = cpu_join(*c3_host, threads)
+        rec["config3_zipf1_16Mx256M"] = {"tuples_per_s": v3, "pairs": n3, "seconds": s3}
+    return rec
 
 
 def main():
@@ -220,6 +235,7 @@ def main():
         "whole_job_roofline_frac": ab["total"] / (ms / 1000.0) / 1e9 / HBM_PEAK_GBS,
         "algorithmic_bytes_total": ab["total"],
     }
+    c2_host = (bk.cpu().numpy(), pk.cpu().numpy()) if not args.no_cpu else None
     del bk, pk
     torch.cuda.empty_cache()
     if args.copy_calibration:
@@ -241,17 +257,27 @@ def main():
             plan_ms = (time.perf_counter() - t0) * 1000.0 / 3
             lms, lph, lcount = run_join_steps(pj, torch, zb, zp, lplan, max(3, args.steps // 2), 1)
             assert lcount == zcount, (lcount, zcount)
-            rec.update({"learned_tuples_per_s": (zb.numel() + zp.numel()) / (lms / 1000.0), "learned_ms": lms,
+            zt = zb.numel() + zp.numel()
+            rec.update({"learned_tuples_per_s": zt / ((lms + plan_ms) / 1000.0),
+                        "learned_ms_including_planning": lms + plan_ms,
+                        "learned_plan_reuse_tuples_per_s": zt / (lms / 1000.0), "learned_ms": lms,
                         "learned_plan": lplan.as_dict(), "learned_fallback_to_plain": fallback,
                         "learned_planning_ms": plan_ms, "learned_phases_ms": lph,
+                        "learned_plan_equals_plain": [lplan.passes, lplan.bits1, lplan.bits2] ==
+                                                     [zplan.passes, zplan.bits1, zplan.bits2],
                         "learned_pair_count_equal_plain": lcount == zcount})
         line["zipf_theta1"] = rec
+        c3_host = (zb.cpu().numpy(), zp.cpu().numpy()) if not args.no_cpu else None
         del zb, zp
         torch.cuda.empty_cache()
     if not args.no_batch:
         line["batched"] = batched_joins_child()
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline()
+        c1b, c1p = G.workload_config1()
+        c1_host = (c1b.cpu().numpy(), c1p.cpu().numpy())
+        del c1b, c1p
+        line["cpu_baseline"] = cpu_baseline(c2_host, c1_host, None if args.no_skew else c3_host,
+                                            min(16, os.cpu_count() or 1))
     print(json.dumps(line), flush=True)
 
 

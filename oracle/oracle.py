@@ -13,9 +13,11 @@ Two layers:
 * numpy restatements of the phase arithmetic below (histogram digits, scans,
   block assignment, scatter checks), each citing the reference line it follows.
 
-``oracle/_ref/libpjref.so`` -- the reference's own CPU code compiled from its
-sources by ``oracle/Makefile`` -- is loaded when present (``ref_lib()``), to pin
-the restatement and to time the reference CPU join.
+The reference's own CPU code is not built here: its headers need the CUDA
+toolkit (``cuda_runtime.h``, ``int2`` and Thrust), which this image lacks, so the
+restatement is pinned by mathematics instead (SMHasher's verification value, C
+division semantics, brute-force and sort-merge joins, closed forms and cuDF's
+published inner-join answers; see ``tests/test_oracle.py``).
 """
 from __future__ import annotations
 
@@ -27,7 +29,6 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = None
-_REF = None
 
 c_i32, c_i64, c_u32, c_u64, c_int, c_dbl, c_vp = (
     ctypes.c_int32, ctypes.c_int64, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int,
@@ -81,33 +82,6 @@ def lib() -> ctypes.CDLL:
             fn.argtypes = args
         _LIB = L
     return _LIB
-
-
-def ref_lib() -> Optional[ctypes.CDLL]:
-    """The reference's own CPU code (oracle/_ref/libpjref.so) or None if not built."""
-    global _REF
-    if _REF is None:
-        path = os.path.join(HERE, "_ref", "libpjref.so")
-        if not os.path.exists(path):
-            return None
-        try:
-            L = ctypes.CDLL(path)
-        except OSError:
-            return None
-        L.pjref_murmur3_i32.restype = c_u32
-        L.pjref_murmur3_i32.argtypes = [c_i32]
-        L.pjref_murmur3_i64.restype = c_u32
-        L.pjref_murmur3_i64.argtypes = [c_i64]
-        L.pjref_fastdiv_div.restype = c_i32
-        L.pjref_fastdiv_div.argtypes = [c_i32, c_i32]
-        L.pjref_fastdiv_mod.restype = c_i32
-        L.pjref_fastdiv_mod.argtypes = [c_i32, c_i32]
-        for nm in ("pjref_inner_join_cpu_i32", "pjref_inner_join_cpu_i64"):
-            fn = getattr(L, nm)
-            fn.restype = c_i64
-            fn.argtypes = [c_vp, c_i64, c_vp, c_i64, c_int, c_vp, c_i64, c_int, ctypes.POINTER(c_dbl)]
-        _REF = L
-    return _REF
 
 
 def _ptr(a: Optional[np.ndarray]):
@@ -272,26 +246,6 @@ def project_to_reference(pairs: np.ndarray, build_keys, probe_pay=None) -> np.nd
     keys = np.asarray(build_keys)[pairs[:, 0]]
     y = pairs[:, 1] if probe_pay is None else np.asarray(probe_pay)[pairs[:, 1]]
     return np.stack([keys.astype(np.int64), y.astype(np.int64)], axis=1)
-
-
-def ref_inner_join_cpu(build_keys, probe_keys, uniq_keys=True, num_threads=0, capacity=None):
-    """The reference's own inner_join_cpu (oracle/_ref), returning its (x, y)
-    pairs (x = key, y = 0), the pair count and its own build+probe seconds.
-    Returns None when oracle/_ref is not built."""
-    L = ref_lib()
-    if L is None:
-        return None
-    bk, kb = _keys_and_width(build_keys)
-    pk, _ = _keys_and_width(probe_keys)
-    if capacity is None:
-        capacity = max(join_count(bk, pk), 1)
-    dt = np.int64 if kb == 8 else np.int32
-    out = np.empty((capacity, 2), dtype=dt)
-    secs = c_dbl(0.0)
-    fn = L.pjref_inner_join_cpu_i64 if kb == 8 else L.pjref_inner_join_cpu_i32
-    n = fn(_ptr(bk), len(bk), _ptr(pk), len(pk), int(bool(uniq_keys)), _ptr(out), capacity, num_threads,
-           ctypes.byref(secs))
-    return out[: min(n, capacity)], n, secs.value
 
 
 # ---------------------------------------------------------------- phases (numpy)

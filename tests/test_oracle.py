@@ -1,10 +1,10 @@
 """Pins of the CPU oracle (runs without a GPU).
 
 Each oracle function is checked against something independent of its own
-formula: the SMHasher verification value, the reference's own compiled code
-(golden vectors in tests/golden/, written by tests/golden/make_golden.py from
-oracle/_ref), C semantics of / and %, brute force on tiny inputs, closed forms
-and cuDF's inner-join known answers.
+formula: the SMHasher verification value, C semantics of / and %, brute force
+on tiny inputs, closed forms and cuDF's inner-join known answers (golden
+vectors in tests/golden/).  The reference's own CPU code is not built in this
+image (its headers need the CUDA toolkit), so nothing here runs it.
 """
 import json
 import math
@@ -31,27 +31,12 @@ def test_murmur3_smhasher_verification(oracle):
     assert oracle.murmur3_32(hashes, 0) == 0xB0F57EE3
 
 
-def test_murmur3_keys_match_reference_code(oracle):
-    g = load("ref_murmur3.json")
-    assert [oracle.murmur3_key(k, 4) for k in g["key4"]] == g["hash4"]
-    assert [oracle.murmur3_key(k, 8) for k in g["key8"]] == g["hash8"]
-
-
 def test_murmur3_key_forms_equal_byte_form(oracle):
     rng = np.random.default_rng(1)
     for k in rng.integers(-2**31, 2**31 - 1, 200).tolist():
         assert oracle.murmur3_key(k, 4) == oracle.murmur3_32(struct.pack("<i", k), 0)
     for k in rng.integers(-2**62, 2**62, 200).tolist():
         assert oracle.murmur3_key(k, 8) == oracle.murmur3_32(struct.pack("<q", k), 0)
-
-
-def test_murmur3_live_reference(oracle):
-    R = oracle.ref_lib()
-    if R is None:
-        pytest.skip("oracle/_ref not built (reference sources absent)")
-    rng = np.random.default_rng(2)
-    for k in rng.integers(-2**31, 2**31 - 1, 500).tolist():
-        assert oracle.murmur3_key(k, 4) == R.pjref_murmur3_i32(k)
 
 
 # ----------------------------------------------------------------- int_fastdiv
@@ -80,14 +65,6 @@ def test_fastdiv_small_exhaustive(oracle):
         f = oracle.fastdiv(d)
         for n in range(-3000, 3000):
             assert oracle.fastdiv_div(n, f) == c_div(n, d)
-
-
-def test_fastdiv_matches_reference_code(oracle):
-    g = load("ref_fastdiv.json")
-    for i, d in enumerate(g["d"]):
-        f = oracle.fastdiv(d)
-        assert [oracle.fastdiv_div(n, f) for n in g["n"]] == g["q"][i]
-        assert [oracle.fastdiv_mod(n, f) for n in g["n"]] == g["r"][i]
 
 
 def test_hash_slot_closed_form(oracle):
@@ -149,27 +126,29 @@ def test_join_count_closed_form(oracle):
     assert oracle.join_count(b, p) == int((cb[:60] * cp[:60]).sum())
 
 
-def test_join_matches_reference_inner_join_cpu(oracle):
-    # Reference output is {x = key, y = 0}; the projection of ours must equal it.
-    for case in load("ref_join.json"):
-        b = np.array(case["build"], dtype=np.int32)
-        p = np.array(case["probe"], dtype=np.int32)
-        pairs = oracle.inner_join_cpu(b, p, uniq_keys=bool(case["uniq"]))
-        assert len(pairs) == case["count"]
-        proj = oracle.project_to_reference(pairs, b)
-        proj[:, 1] = 0
-        assert np.array_equal(sorted_pairs(proj), sorted_pairs(case["xy"]))
+def _reference_form_by_definition(b, p):
+    """{(R.key[b], S.payload[p]) : R.key[b] = S.key[p]} by nested loops: the
+    reference's output form {x = key, y = probe payload}
+    (src/hash_join_kernels.cuh:155-164), with row-id payloads."""
+    return sorted_pairs([(int(kb), j) for kb in b.tolist() for j, kp in enumerate(p.tolist()) if kb == kp])
+
+
+@pytest.mark.parametrize("uniq", [True, False])
+def test_projection_equals_reference_form(oracle, uniq):
+    rng = np.random.default_rng(13 if uniq else 14)
+    b = (rng.permutation(300)[:200] if uniq else rng.integers(0, 150, 200)).astype(np.int32)
+    p = rng.integers(0, 450, 700).astype(np.int32)
+    pairs = oracle.inner_join_cpu(b, p, uniq_keys=uniq)
+    assert np.array_equal(sorted_pairs(oracle.project_to_reference(pairs, b)), _reference_form_by_definition(b, p))
 
 
 def test_project_to_reference_planted_mistake_is_caught(oracle):
-    case = load("ref_join.json")[0]
-    b = np.array(case["build"], dtype=np.int32)
-    p = np.array(case["probe"], dtype=np.int32)
+    rng = np.random.default_rng(15)
+    b = rng.permutation(300)[:200].astype(np.int32)
+    p = rng.integers(0, 450, 700).astype(np.int32)
     pairs = oracle.inner_join_cpu(b, p, uniq_keys=True).copy()
     pairs[0, 0] = (pairs[0, 0] + 1) % len(b)  # planted: wrong build row
-    proj = oracle.project_to_reference(pairs, b)
-    proj[:, 1] = 0
-    assert not np.array_equal(sorted_pairs(proj), sorted_pairs(case["xy"]))
+    assert not np.array_equal(sorted_pairs(oracle.project_to_reference(pairs, b)), _reference_form_by_definition(b, p))
 
 
 # ----------------------------------------------------------------- generators
