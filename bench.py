@@ -136,6 +136,11 @@ def batched_joins_child(timeout_s=600):
     return json.loads(lines[-1])
 
 
+def log(msg):
+    """Progress line on stderr (the JSON result is the only stdout line)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def cpu_info():
     model = "unknown"
     try:
@@ -204,6 +209,7 @@ def main():
     torch.cuda.synchronize()
     plan = pj.plan_default(bk.numel(), pk.numel())
     ms, ph, count = run_join_steps(pj, torch, bk, pk, plan, args.steps, args.warmup)
+    log(f"64M x 256M join: {ms:.3f} ms per step, {count} pairs")
     n_r, n_s = bk.numel(), pk.numel()
     ab = algorithmic_bytes(n_r, n_s, count, plan.passes)
     kern = {"pass1": ph["pass1_ms"], "pass2": ph["pass2_ms"], "join": ph["join_ms"],
@@ -267,12 +273,15 @@ def main():
                                                      [zplan.passes, zplan.bits1, zplan.bits2],
                         "learned_pair_count_equal_plain": lcount == zcount})
         line["zipf_theta1"] = rec
+        log(f"Zipf(1.0) join: plain {zms:.3f} ms")
         c3_host = (zb.cpu().numpy(), zp.cpu().numpy()) if not args.no_cpu else None
         del zb, zp
         torch.cuda.empty_cache()
     if not args.no_batch:
         line["batched"] = batched_joins_child()
+        log("batched joins done")
     if not args.no_cpu:
+        log("CPU baseline (inner_join_cpu on the host cores) ...")
         c1b, c1p = G.workload_config1()
         c1_host = (c1b.cpu().numpy(), c1p.cpu().numpy())
         del c1b, c1p

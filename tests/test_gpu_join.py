@@ -47,18 +47,28 @@ def test_known_answers(pj, oracle, torch):
         assert np.array_equal(sorted_np(got.cpu().numpy()), sorted_np(c["pairs"])), c["name"]
 
 
-def test_reference_join_vectors_projection(pj, oracle, torch):
-    # the reference's own inner_join_cpu output {key, 0}, compared through the projection
-    with open(os.path.join(GOLDEN, "ref_join.json")) as f:
-        cases = json.load(f)
-    for c in cases:
-        bk = torch.tensor(c["build"], dtype=torch.int32, device="cuda")
-        pk = torch.tensor(c["probe"], dtype=torch.int32, device="cuda")
-        pairs = pj.partitioned_join(bk, pk, plan={"passes": 1, "bits1": 4, "unique_build_keys": c["uniq"]})
-        assert pairs.shape[0] == c["count"]
-        xy = pj.project_reference(bk, pairs).cpu().numpy()
-        xy[:, 1] = 0
-        assert np.array_equal(sorted_np(xy), sorted_np(c["xy"]))
+def test_reference_output_projection(pj, oracle, torch):
+    """The reference's output form {key, probe payload} (src/hash_join_kernels.cuh:174-176)
+    is the projection (R.key[b], S.payload[p]) of the pairs: checked against the
+    oracle's projection of its own join, with duplicates on both sides and the
+    reference's zero probe payload (main.cu's driver) as well as a payload column."""
+    rng = np.random.default_rng(0x9E0)
+    for nb, ns, span, uniq in [(50, 200, 40, 0), (1000, 5000, 3000, 0), (4096, 4096, 1 << 30, 1)]:
+        bk = rng.integers(-span, span, nb).astype(np.int32)
+        if uniq:
+            bk = rng.permutation(np.arange(nb, dtype=np.int32) * 7 - 11)
+        pk = rng.integers(-span, span, ns).astype(np.int32)
+        pp = rng.integers(-2**31, 2**31 - 1, ns).astype(np.int32)
+        for pay in (None, pp):
+            dbk, dpk = torch.from_numpy(bk).cuda(), torch.from_numpy(pk).cuda()
+            dpp = None if pay is None else torch.from_numpy(pay).cuda()
+            pairs = pj.partitioned_join(dbk, dpk, None, dpp, plan={"passes": 1, "bits1": 4, "unique_build_keys": uniq})
+            exp = oracle.inner_join_cpu(bk, pk, None, pay, uniq_keys=bool(uniq))
+            assert pairs.shape[0] == len(exp)
+            # both sides' pairs hold build row ids and probe payloads (row ids without a column)
+            xy = pj.project_reference(dbk, pairs).cpu().numpy()
+            ref = oracle.project_to_reference(exp, bk)
+            assert np.array_equal(sorted_np(xy), sorted_np(ref))
 
 
 def test_config1_exact(pj, oracle, torch):
