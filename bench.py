@@ -41,21 +41,21 @@ def algorithmic_bytes(n_r, n_s, n_out, passes, w=8):
     }
 
 
-def run_join_steps(pj, torch, bk, pk, plan, steps, warmup):
+def run_join_steps(pj, torch, bk, pk, plan, steps, warmup, bp=None, pp=None, n_cap=None):
     """warmup + timed steps of the async join; returns (ms/step, phase means, count)."""
-    n_cap = pk.numel() + 1024
+    n_cap = n_cap if n_cap is not None else pk.numel() + 1024
     out = torch.empty((n_cap, 2), dtype=torch.int32, device="cuda")
     ws_bytes = pj.workspace_size(bk.numel(), pk.numel(), bk.element_size(), plan)
     ws = torch.empty(ws_bytes, dtype=torch.uint8, device="cuda")
     for _ in range(warmup):
-        d_count = pj.partitioned_join_async(bk, pk, out, plan=plan, workspace=ws)
+        d_count = pj.partitioned_join_async(bk, pk, out, bp, pp, plan=plan, workspace=ws)
     torch.cuda.synchronize()
     pj.profile_enable(True)
     phases = []
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
-        d_count = pj.partitioned_join_async(bk, pk, out, plan=plan, workspace=ws)
+        d_count = pj.partitioned_join_async(bk, pk, out, bp, pp, plan=plan, workspace=ws)
         phases.append(pj.profile_read())
     torch.cuda.synchronize()
     t1 = time.perf_counter()
@@ -136,6 +136,63 @@ def batched_joins_child(timeout_s=600):
     return json.loads(lines[-1])
 
 
+def variant_records(pj, torch, steps):
+    """The same join shape with 64-bit keys, with explicit payload columns and
+    with duplicate build keys: tuples/s and the whole-job roofline fraction
+    with w = key + payload bytes."""
+    G = pj.generators
+    recs = {}
+    n_r, n_s = 1 << 26, 1 << 28
+    L = pj.lib()
+    s = torch.cuda.current_stream().cuda_stream
+    # 64-bit keys: uniform random build keys (unique with overwhelming probability), half the probes match
+    bk = torch.empty(n_r, dtype=torch.int64, device="cuda")
+    pk = torch.empty(n_s, dtype=torch.int64, device="cuda")
+    pj._lib.check(L.pj_gen_build_i64(bk.data_ptr(), n_r, 0x64B, s))
+    pj._lib.check(L.pj_gen_probe_i64(pk.data_ptr(), n_s, n_r, 0x64B, 0x64F, 0.5, s))
+    plan = pj.plan_default(n_r, n_s, 8)
+    ms, ph, count = run_join_steps(pj, torch, bk, pk, plan, steps, 1)
+    ab = algorithmic_bytes(n_r, n_s, count, plan.passes, w=12)
+    recs["int64_keys"] = {"workload": "64M uniform 64-bit build keys x 256M probes, 50% match, row-id payloads",
+                          "ms": ms, "tuples_per_s": (n_r + n_s) / (ms / 1000.0), "pairs": count,
+                          "whole_job_roofline_frac": ab["total"] / (ms / 1000.0) / 1e9 / HBM_PEAK_GBS,
+                          "plan": plan.as_dict(), "phases_ms": ph}
+    log(f"int64 64M x 256M: {ms:.3f} ms")
+    del bk, pk
+    torch.cuda.empty_cache()
+    # payload columns: the 64M x 256M workload with random int32 payloads on both sides
+    bk, pk = G.workload_config2()
+    g = torch.Generator(device="cuda").manual_seed(0xBA1)
+    bp = torch.randint(-2**31, 2**31 - 1, (n_r,), generator=g, device="cuda", dtype=torch.int32)
+    pp = torch.randint(-2**31, 2**31 - 1, (n_s,), generator=g, device="cuda", dtype=torch.int32)
+    plan = pj.plan_default(n_r, n_s, 4)
+    ms, ph, count = run_join_steps(pj, torch, bk, pk, plan, steps, 1, bp, pp)
+    ab = algorithmic_bytes(n_r, n_s, count, plan.passes, w=8)
+    recs["payload_columns"] = {"workload": "64M x 256M int32 (the headline keys) with int32 payload columns on both "
+                                           "sides", "ms": ms, "tuples_per_s": (n_r + n_s) / (ms / 1000.0),
+                               "pairs": count,
+                               "whole_job_roofline_frac": ab["total"] / (ms / 1000.0) / 1e9 / HBM_PEAK_GBS,
+                               "phases_ms": ph}
+    log(f"payload columns 64M x 256M: {ms:.3f} ms")
+    del bp, pp, pk
+    # duplicate build keys: every value of [0, 2^24) four times; probes uniform over [0, 2^25)
+    bk = torch.div(bk, 4, rounding_mode="floor")
+    pk = torch.randint(0, 1 << 25, (1 << 26,), generator=g, device="cuda", dtype=torch.int32)
+    plan = pj.plan_default(bk.numel(), pk.numel(), 4)
+    ms, ph, count = run_join_steps(pj, torch, bk, pk, plan, steps, 1, n_cap=4 * pk.numel() + 1024)
+    t = bk.numel() + pk.numel()
+    ab = algorithmic_bytes(bk.numel(), pk.numel(), count, plan.passes, w=8)
+    recs["duplicate_build_keys"] = {"workload": "64M build keys (each of 2^24 values 4 times) x 64M probes "
+                                                "uniform over [0, 2^25)", "ms": ms, "tuples_per_s": t / (ms / 1000.0),
+                                    "pairs": count, "pairs_per_s": count / (ms / 1000.0),
+                                    "whole_job_roofline_frac": ab["total"] / (ms / 1000.0) / 1e9 / HBM_PEAK_GBS,
+                                    "phases_ms": ph}
+    log(f"duplicate keys 64M x 64M: {ms:.3f} ms")
+    del bk, pk
+    torch.cuda.empty_cache()
+    return recs
+
+
 def log(msg):
     """Progress line on stderr (the JSON result is the only stdout line)."""
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
@@ -188,6 +245,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-skew", action="store_true", help="skip the Zipf(1.0) comparison")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-joins graph measurement")
+    ap.add_argument("--no-variants", action="store_true",
+                    help="skip the 64-bit-key, payload-column and duplicate-key records")
     ap.add_argument("--batch-only", action="store_true",
                     help="print only the batched-joins record (the child process of a full run)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -277,6 +336,8 @@ def main():
         c3_host = (zb.cpu().numpy(), zp.cpu().numpy()) if not args.no_cpu else None
         del zb, zp
         torch.cuda.empty_cache()
+    if not args.no_variants:
+        line["variants"] = variant_records(pj, torch, max(3, args.steps // 2))
     if not args.no_batch:
         line["batched"] = batched_joins_child()
         log("batched joins done")

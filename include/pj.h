@@ -16,6 +16,10 @@
  *  - Tables are SoA like the reference (src/common.cuh:23-43): a key column of
  *    int32 or int64 (key_bytes 4 or 8) and an optional int32 payload column.
  *    A NULL payload means "row id" (payload[i] == i).  Inputs are read-only.
+ *    Payload columns are int32 for both key widths.  The reference's 8-byte
+ *    key build carries 8-byte payloads (value_type long long,
+ *    src/common.cuh:26-27); here such columns are joined by row id (NULL
+ *    payload) and fetched with pj_gather, which moves 4- or 8-byte columns.
  *  - Output pairs are {int32 build, int32 probe} with the int2 layout: the
  *    build and probe payloads of every matching (build row, probe row), i.e.
  *    (build_idx, probe_idx) when payloads are row ids (the cuDF gather-map

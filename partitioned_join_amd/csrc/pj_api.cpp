@@ -63,17 +63,28 @@ struct Carver {
   }
 };
 
+bool deep_plan(const PartParams& pp) { return pp.passes == 2 && pp.B > kMaxCombinedBits; }
+
+// Deep plans keep only the [segment][pass-1 digit] matrix (in mat) and count
+// the pass-2 digits per pass-2 item (h2) after pass 1.
 static void carve_table(Carver& c, int64_t n, int key_bytes, const PartParams& pp, bool two_pass, PartTable* T) {
   const size_t tb = size_t(tuple_bytes(key_bytes));
+  const bool deep = deep_plan(pp);
   T->n = uint32_t(n);
   T->nseg = choose_nseg(n);
   T->buf1 = c.take(size_t(n) * tb + 64);
   T->buf2 = two_pass ? c.take(size_t(n) * tb + 64) : nullptr;
-  T->mat = static_cast<uint32_t*>(c.take(size_t(T->nseg) * pp.P * 4));
-  T->matc = two_pass ? static_cast<uint32_t*>(c.take(size_t(T->nseg) * pp.F1 * 4)) : nullptr;
+  T->mat = static_cast<uint32_t*>(c.take(size_t(T->nseg) * (deep ? pp.F1 : pp.P) * 4));
+  T->matc = two_pass && !deep ? static_cast<uint32_t*>(c.take(size_t(T->nseg) * pp.F1 * 4)) : nullptr;
   T->tot = static_cast<uint32_t*>(c.take(size_t(pp.P) * 4));
   T->totc = two_pass ? static_cast<uint32_t*>(c.take(size_t(pp.F1) * 4)) : nullptr;
   T->off = static_cast<uint32_t*>(c.take(size_t(pp.P + 1) * 4));
+  T->offc = T->h2 = T->ipre = nullptr;
+  if (deep) {
+    T->offc = static_cast<uint32_t*>(c.take(size_t(pp.F1 + 1) * 4));
+    T->h2 = static_cast<uint32_t*>(c.take(size_t(deep_item_capacity(n, key_bytes, pp)) * pp.F2 * 4));
+    T->ipre = static_cast<uint32_t*>(c.take(size_t((pp.F1 + kNumXcd - 1) / kNumXcd * kNumXcd + 1) * 4));
+  }
 }
 
 static void carve_join(Carver& c, const PartParams& pp, uint64_t desc_cap, Layout* L) {
@@ -147,7 +158,7 @@ static void plain_plan(int64_t nR, int key_bytes, pj_plan* plan) {
   const double cap = double(join_table_capacity(key_bytes) / 2 - 1);
   const double target = cap < 1536.0 ? cap : 1536.0;  // build tuples per chunk at factor f
   int B = 1;
-  while (B < kMaxTotalBits && f * double(nR) / double(int64_t(1) << B) > target) ++B;
+  while (B < kMaxCombinedBits && f * double(nR) / double(int64_t(1) << B) > target) ++B;
   plan->max_partition_size_factor = f;
   plan->unique_build_keys = 0;
   plan->probe_slice = 0;
@@ -268,14 +279,36 @@ static pj_status run_join(const pj_table* build, const pj_table* probe, const pj
     g_prof.created = true;
   }
   prof_mark(0, s);
-  PJ_HIP(launch_hist(L, kb, s));
-  prof_mark(1, s);
-  PJ_HIP(launch_colscan(L, s));
-  PJ_HIP(launch_offsets(L, s));
-  prof_mark(2, s);
-  PJ_HIP(launch_pass1(L, kb, s));
-  prof_mark(3, s);
-  if (pp.passes == 2) PJ_HIP(launch_pass2(L, kb, s));
+  if (deep_plan(pp)) {
+    // Pass 1 as a one-pass partition on bits [b2, b2 + b1) into offc, then
+    // the pass-2 digit counts per item over its output.
+    Layout L1 = L;
+    L1.pp = PartParams{pp.b1, pp.b1, 0, pp.F1, pp.F1, 1, 1, pp.b2};
+    for (int t = 0; t < 2; ++t) {
+      L1.t[t].off = L.t[t].offc;
+      L1.t[t].tot = L.t[t].totc;
+    }
+    PJ_HIP(launch_hist(L1, kb, s));
+    prof_mark(1, s);
+    PJ_HIP(launch_colscan(L1, s));
+    PJ_HIP(launch_offsets(L1, s));
+    prof_mark(2, s);
+    PJ_HIP(launch_pass1(L1, kb, s));
+    prof_mark(3, s);
+    PJ_HIP(launch_hist2(L, kb, s));
+    PJ_HIP(launch_segscan(L, s));
+    PJ_HIP(launch_offsets(L, s));
+    PJ_HIP(launch_pass2(L, kb, s));
+  } else {
+    PJ_HIP(launch_hist(L, kb, s));
+    prof_mark(1, s);
+    PJ_HIP(launch_colscan(L, s));
+    PJ_HIP(launch_offsets(L, s));
+    prof_mark(2, s);
+    PJ_HIP(launch_pass1(L, kb, s));
+    prof_mark(3, s);
+    if (pp.passes == 2) PJ_HIP(launch_pass2(L, kb, s));
+  }
   prof_mark(4, s);
   PJ_HIP(launch_join_plan(L, base, slice, s));
   prof_mark(5, s);
@@ -396,7 +429,7 @@ static void one_table_layout(Carver& c, const pj_table* t, const PartParams& pp,
 }
 
 size_t pj_histogram_workspace_size(int32_t total_bits) {
-  if (total_bits < 1 || total_bits > kMaxTotalBits) return 0;
+  if (total_bits < 1 || total_bits > kMaxCombinedBits) return 0;
   const size_t P = size_t(1) << total_bits;
   return size_t(kMaxSegPerXcd * kNumXcd) * (P + 4096) * 4 + 4 * 256 + P * 4 + 2 * 4096 * 8;
 }
@@ -405,7 +438,7 @@ pj_status pj_combined_histogram(const pj_table* table, int32_t total_bits, int32
                                 uint32_t* d_pass1, void* workspace, size_t workspace_bytes, hipStream_t stream) {
   pj_status st = check_table(table, "input");
   if (st != PJ_OK) return st;
-  if (total_bits < 1 || total_bits > kMaxTotalBits) return fail(PJ_ERR_INVALID_ARG, "total_bits must be in [1, 16]");
+  if (total_bits < 1 || total_bits > kMaxCombinedBits) return fail(PJ_ERR_INVALID_ARG, "total_bits must be in [1, 16]");
   if (bits2 < 0 || total_bits - bits2 < 1 || total_bits - bits2 > 12)
     return fail(PJ_ERR_INVALID_ARG, "pass-1 bits (total_bits - bits2) must be in [1, 12]");
   if (!d_fine || !d_pass1) return fail(PJ_ERR_INVALID_ARG, "output histogram pointer is NULL");
@@ -512,7 +545,7 @@ pj_status pj_partition_pass(const pj_table* table, int32_t shift, int32_t bits, 
 
 size_t pj_partition2_workspace_size(int64_t n, int32_t bits1, int32_t bits2) {
   (void)n;
-  if (bits1 < 1 || bits1 > 12 || bits2 < 1 || bits2 > 12 || bits1 + bits2 > kMaxTotalBits) return 0;
+  if (bits1 < 1 || bits1 > 12 || bits2 < 1 || bits2 > 12 || bits1 + bits2 > kMaxCombinedBits) return 0;
   const size_t P = size_t(1) << (bits1 + bits2), F1 = size_t(1) << bits1;
   return size_t(kNumXcd) * (P + F1) * 4 + P * 4 + 2 * 4096 * 8 + 4 * 256;
 }
@@ -522,7 +555,7 @@ pj_status pj_partition_pass2(const void* d_tuples_in, int64_t n, int32_t key_byt
                              size_t workspace_bytes, hipStream_t stream) {
   if (key_bytes != 4 && key_bytes != 8) return fail(PJ_ERR_INVALID_ARG, "key_bytes must be 4 or 8");
   if (n < 0 || n >= (int64_t(1) << 31)) return fail(PJ_ERR_INVALID_ARG, "n outside [0, 2^31)");
-  if (bits1 < 1 || bits1 > 12 || bits2 < 1 || bits2 > 12 || bits1 + bits2 > kMaxTotalBits)
+  if (bits1 < 1 || bits1 > 12 || bits2 < 1 || bits2 > 12 || bits1 + bits2 > kMaxCombinedBits)
     return fail(PJ_ERR_INVALID_ARG, "bits1, bits2 must be in [1, 12] with bits1 + bits2 <= 16");
   if (!d_offsets1 || !d_offsets2 || (n > 0 && (!d_tuples_in || !d_tuples_out)))
     return fail(PJ_ERR_INVALID_ARG, "NULL argument");
@@ -577,7 +610,7 @@ pj_status pj_assign_blocks(const uint32_t* d_offsets, int64_t total, int32_t num
 }
 
 size_t pj_join_workspace_size(int32_t total_bits) {
-  if (total_bits < 1 || total_bits > kMaxTotalBits) return 0;
+  if (total_bits < 1 || total_bits > kMaxCombinedBits) return 0;
   return join_generic_ws_bytes(total_bits) + 4096;
 }
 
@@ -586,7 +619,7 @@ pj_status pj_join_partitions(const void* d_build_tuples, const uint32_t* d_build
                              int64_t probe_slice, int32_t unique_build_keys, pj_pair* out, int64_t out_capacity,
                              int64_t* d_out_count, void* workspace, size_t workspace_bytes, hipStream_t stream) {
   if (key_bytes != 4 && key_bytes != 8) return fail(PJ_ERR_INVALID_ARG, "key_bytes must be 4 or 8");
-  if (total_bits < 1 || total_bits > kMaxTotalBits) return fail(PJ_ERR_INVALID_ARG, "total_bits must be in [1, 16]");
+  if (total_bits < 1 || total_bits > kMaxCombinedBits) return fail(PJ_ERR_INVALID_ARG, "total_bits must be in [1, 16]");
   if (!d_build_offsets || !d_probe_offsets || !d_out_count) return fail(PJ_ERR_INVALID_ARG, "NULL argument");
   if (out_capacity < 0 || (out_capacity > 0 && !out)) return fail(PJ_ERR_INVALID_ARG, "bad output");
   if (!workspace || workspace_bytes < pj_join_workspace_size(total_bits))
@@ -838,7 +871,7 @@ pj_status pj_plan_decode(const float* logits, const uint32_t* cand_sample_counts
   } else {  // three passes are not implemented: two passes with the predicted bits
     learned.passes = 2;
     learned.bits1 = b1;
-    learned.bits2 = b1 + b2 > kMaxTotalBits ? kMaxTotalBits - b1 : b2;
+    learned.bits2 = b1 + b2 > kMaxCombinedBits ? kMaxCombinedBits - b1 : b2;
   }
   learned.source = 1;
   // a plan whose build chunks would not fit the LDS table is never taken

@@ -22,7 +22,9 @@ constexpr int64_t kOutChunk = 16384;  // output pairs claimed per global allocat
 constexpr int kJoinGrid = 512;        // persistent join workgroups (2 per CU)
 constexpr int kPass2WgPerXcd = 16;    // pass-2 persistent workgroups per XCD and table (one per CU)
 constexpr int kPass2ItemTiles = 4;    // pass-2 items hold about this many full tiles
-constexpr int kMaxTotalBits = 16;     // combined histogram limit (65536 partitions)
+constexpr int kMaxCombinedBits = 16;  // combined histogram limit (65536 partitions in LDS)
+constexpr int kMaxTotalBits = 24;     // two passes of at most 12 bits; above kMaxCombinedBits the
+                                      // second pass counts its digits over pass 1's output ("deep" plans)
 constexpr int kMaxRadix = 4096;       // largest fan-out of one pass (12 bits)
 constexpr int kMaxSegPerXcd = 32;    // segments per XCD and table at most: the [segment][digit] matrix the histogram writes and the column scans read shrinks with it, while pass 1 (one workgroup per segment) still has two per CU (64M x 256M: 64 -> 4.41 ms, 32 -> 4.17, 16 -> 4.39)
 
@@ -39,6 +41,11 @@ struct PartTable {
   uint32_t* tot;         // [P] fine totals
   uint32_t* totc;        // [F1] pass-1 totals (2-pass)
   uint32_t* off;         // [P+1] final partition starts, off[P] = n
+  // deep plans (total bits > kMaxCombinedBits) only:
+  uint32_t* offc;        // [F1+1] pass-1 partition starts
+  uint32_t* h2;          // [items][F2] pass-2 digit counts per pass-2 item, then their exclusive scan
+                         // over the items of the same pass-1 partition
+  uint32_t* ipre;        // [RN+1] pass-2 items before each pass-1 partition rank
 };
 
 struct PartParams {
@@ -72,6 +79,7 @@ struct JoinArgs {
   uint64_t* wg_last;       // [grid] (chunk << 32 | fill) or ~0 when no chunk; fill <= kOutChunk
   int table_cap;           // LDS slots available for the build table (a multiple of 4)
   int bucket_slots;        // keys per 16-byte bucket: 4 (4-byte keys) or 2 (8-byte keys)
+  uint32_t cap_buckets;    // largest power-of-two bucket count that fits table_cap
   uint32_t nr, ns;         // table sizes (bounds for the item ranges)
   const uint4* desc;       // expanded work item descriptors (2 uint4 per item)
   uint64_t desc_cap;       // items with a descriptor; later items are decoded by search
@@ -105,6 +113,8 @@ struct Layout {
 };
 
 int tuple_bytes(int key_bytes);
+// Two passes over more partitions than the combined histogram holds.
+bool deep_plan(const PartParams& pp);
 uint32_t choose_nseg(int64_t n);
 size_t layout_join(int64_t nR, int64_t nS, int key_bytes, const PartParams& pp, int base, int64_t slice,
                    char* ws, Layout* L);
@@ -117,9 +127,17 @@ hipError_t launch_colscan(const Layout& L, hipStream_t s);
 hipError_t launch_offsets(const Layout& L, hipStream_t s);
 hipError_t launch_pass1(const Layout& L, int key_bytes, hipStream_t s);
 hipError_t launch_pass2(const Layout& L, int key_bytes, hipStream_t s);
+// Deep plans: pass-2 digit counts per item over pass 1's output (h2, ipre),
+// their scan over each pass-1 partition's items and the fine totals (tot).
+hipError_t launch_hist2(const Layout& L, int key_bytes, hipStream_t s);
+hipError_t launch_segscan(const Layout& L, hipStream_t s);
+// Pass-2 items of a deep plan: at most this many per table.
+uint64_t deep_item_capacity(int64_t n, int key_bytes, const PartParams& pp);
 hipError_t launch_join_plan(const Layout& L, int base, int64_t slice, hipStream_t s);
+// digit_sentinel: the tuples were partitioned here by their low key bits
+// (false for caller-partitioned tuples, which may not be).
 hipError_t launch_join(const Layout& L, int key_bytes, int base, int64_t slice, int unique, pj_pair* out,
-                       int64_t capacity, hipStream_t s);
+                       int64_t capacity, hipStream_t s, bool digit_sentinel = true);
 hipError_t launch_defrag(const Layout& L, pj_pair* out, int64_t capacity, int64_t* d_count, hipStream_t s);
 
 // Generic join over externally partitioned tuples (phase API).
@@ -132,13 +150,16 @@ size_t join_generic_ws_bytes(int B);
 // the rest of kJoinLdsBytes); the build chunk is at most half of it.
 int join_table_capacity(int key_bytes);
 
-// Slots of the table of a build chunk of c tuples: 8c + 2 rounded up to whole
-// 16-byte key buckets (occupancy <= 1/8), at most cap (a multiple of 4).
-// c <= cap/2 - 1 keeps at least 2c + 2, so every probe walk ends at an empty
-// slot.
-__host__ __device__ inline uint32_t join_table_slots(uint32_t c, uint32_t cap) {
-  const uint64_t t = (8ull * c + 2ull + 3ull) & ~3ull;
-  return t < cap ? uint32_t(t) : cap;
+// log2 of the bucket count of the table of a build chunk of c tuples: the
+// power of two giving at least 4c slots (occupancy <= 1/4), at most
+// cap_buckets.  bs = slots per bucket.  A chunk is at most table_cap / 2 - 1
+// tuples and cap_buckets * bs > table_cap / 2, so every probe walk ends at an
+// empty slot.
+__host__ __device__ inline uint32_t join_bucket_log2(uint32_t c, uint32_t bs, uint32_t cap_buckets) {
+  const uint32_t want = (4u * c + bs - 1u) / bs;  // buckets for 4c slots
+  uint32_t lg = want <= 1u ? 0u : 32u - uint32_t(__builtin_clz(want - 1u));
+  const uint32_t cap_lg = 31u - uint32_t(__builtin_clz(cap_buckets));
+  return lg < cap_lg ? lg : cap_lg;
 }
 
 }  // namespace pj

@@ -7,11 +7,13 @@
 //                handle_skewed_data reads the block count back to the host,
 //                src/skewed_data_handler.cuh:83, and splits only the build side).
 //   k_join       persistent workgroups; per item: LDS open-addressing table of
-//                8*|chunk| + 2 slots (occupancy <= 1/8) in 16-byte key buckets,
-//                MurmurHash3 seed 0, bucket = |(int)h % buckets| with
-//                int_fastdiv, linear probing with LDS CAS from the bucket's
-//                first slot (src/hash_join_kernels.cuh:84-129), then the probe
-//                slice walks the table (:134-181).  Slots fill each bucket in
+//                a power-of-two number of 16-byte key buckets with at least
+//                4*|chunk| slots (occupancy <= 1/4), bucket = top bits of a
+//                multiplicative hash of the key, linear probing with LDS CAS
+//                from the bucket's first slot (the reference hashes with
+//                MurmurHash3 and slots with int_fastdiv,
+//                src/hash_join_kernels.cuh:84-129; the slot never changes the
+//                result), then the probe slice walks the table (:134-181).  Slots fill each bucket in
 //                order, so a probe reads one whole bucket per step (one 16-byte
 //                LDS read) and stops at a match or at an empty slot: nearly
 //                every probe ends in its first bucket.  Probes of one thread
@@ -20,9 +22,13 @@
 //                staged in LDS and flushed into output chunks of kOutChunk
 //                pairs that each workgroup claims with one global atomic (a
 //                same-address device atomic costs ~12 ns on MI355X, so claims
-//                stay rare).  The empty-slot sentinel is a key whose low bits
-//                differ from the partition digit, so every real key value can
-//                be joined (the reference reserves -1, :20,86,106).
+//                stay rare).  The empty-slot sentinel is a value absent from
+//                the build chunk, so every key value can be joined (the
+//                reference reserves -1, :20,86,106): in the join pipeline a
+//                key whose low bits differ from the partition digit; over
+//                caller-partitioned tuples one found per chunk (below the
+//                chunk's minimum, above its maximum, or a gap).  A probe key
+//                equal to the sentinel has no match and does not walk.
 //   k_defrag_*   the last chunk of each workgroup is partly filled; pairs that
 //                lie past the total are moved into those holes so the output
 //                is dense in [0, total).
@@ -147,13 +153,25 @@ __device__ __forceinline__ uint4 bucket_fill(uint64_t k) {
 __device__ __forceinline__ uint32_t uniform(uint32_t x) { return __builtin_amdgcn_readfirstlane(x); }
 
 // One work item: partition q, build chunk [rlo, rhi), probe slice [slo, shi);
-// the LDS table of the chunk has fd.d = join_table_slots(rhi - rlo) /
-// bucket_slots buckets.
-// Stored as two uint4 words: {q, rlo, rhi, slo}, {shi, magic, shift, add_sign}.
+// the LDS table of the chunk has 2^lg buckets (join_bucket_log2).
+// Stored as two uint4 words: {q, rlo, rhi, slo}, {shi, lg, 0, 0}.
 struct ItemDesc {
-  uint32_t q, rlo, rhi, slo, shi, valid;
-  FastDiv fd;
+  uint32_t q, rlo, rhi, slo, shi, valid, lg;
 };
+
+// Bucket hash of the LDS table: a multiplicative (Fibonacci) hash whose top
+// bits depend on every key bit, so keys that share their low bits (one
+// partition's) still spread over the buckets.  One 32-bit multiply (two for
+// 8-byte keys).
+__device__ __forceinline__ uint32_t table_hash(uint32_t k) { return k * 0x9E3779B1u; }
+__device__ __forceinline__ uint32_t table_hash(uint64_t k) {
+  return (uint32_t(k) ^ (uint32_t(k >> 32) * 0x85EBCA77u)) * 0x9E3779B1u;
+}
+// First slot of key k's bucket in a table of 2^lg buckets of bs slots.
+template <uint32_t BS, typename K>
+__device__ __forceinline__ uint32_t first_slot(K k, uint32_t lg) {
+  return BS * ((table_hash(k) >> ((32u - lg) & 31u)) & ((1u << lg) - 1u));
+}
 
 // Per-thread decode of work item `item` < total: binary search of its
 // partition in item_scan, then its chunk and slice.
@@ -176,7 +194,7 @@ __device__ __forceinline__ ItemDesc decode_item(const JoinArgs& A, uint64_t item
   const uint64_t slo64 = uint64_t(s0) + sl * uint64_t(A.slice);
   d.slo = uint32_t(min(uint64_t(s1), slo64));
   d.shi = uint32_t(min(uint64_t(s1), slo64 + uint64_t(A.slice)));
-  d.fd = fastdiv_make(int32_t(join_table_slots(d.rhi - d.rlo, uint32_t(A.table_cap)) / uint32_t(A.bucket_slots)));
+  d.lg = join_bucket_log2(d.rhi - d.rlo, uint32_t(A.bucket_slots), A.cap_buckets);
   d.valid = 1;
   return d;
 }
@@ -189,7 +207,7 @@ __global__ __launch_bounds__(256) void k_items_expand(JoinArgs A, uint4* desc, u
   for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < lim; i += uint64_t(gridDim.x) * blockDim.x) {
     const ItemDesc d = decode_item(A, i);
     desc[2 * i] = make_uint4(d.q, d.rlo, d.rhi, d.slo);
-    desc[2 * i + 1] = make_uint4(d.shi, uint32_t(d.fd.magic), uint32_t(d.fd.shift), uint32_t(d.fd.add_sign));
+    desc[2 * i + 1] = make_uint4(d.shi, d.lg, 0u, 0u);
   }
 }
 
@@ -222,10 +240,7 @@ __device__ __forceinline__ ItemDesc desc_from_raw(const JoinArgs& A, const uint4
   d.rhi = valid ? a.z : 0u;
   d.slo = valid ? a.w : 0u;
   d.shi = valid ? b.x : 0u;
-  d.fd.magic = valid ? int32_t(b.y) : 0;
-  d.fd.shift = valid ? int32_t(b.z) : -1;
-  d.fd.add_sign = valid ? int32_t(b.w) : 0;
-  d.fd.d = valid ? int32_t(join_table_slots(d.rhi - d.rlo, uint32_t(A.table_cap)) / uint32_t(A.bucket_slots)) : 2;
+  d.lg = valid ? b.y : 0u;
   d.valid = valid ? 1u : 0u;
   if (valid && item >= A.desc_cap) d = decode_item(A, item);  // beyond the expanded descriptors
   d.q = uniform(d.q);
@@ -234,10 +249,7 @@ __device__ __forceinline__ ItemDesc desc_from_raw(const JoinArgs& A, const uint4
   d.slo = uniform(d.slo);
   d.shi = uniform(d.shi);
   d.valid = uniform(d.valid);
-  d.fd.d = int32_t(uniform(uint32_t(d.fd.d)));
-  d.fd.magic = int32_t(uniform(uint32_t(d.fd.magic)));
-  d.fd.shift = int32_t(uniform(uint32_t(d.fd.shift)));
-  d.fd.add_sign = int32_t(uniform(uint32_t(d.fd.add_sign)));
+  d.lg = uniform(d.lg);
   return d;
 }
 
@@ -252,7 +264,11 @@ __device__ __forceinline__ ItemDesc desc_from_raw(const JoinArgs& A, const uint4
 // atomic per wave and step.  When the stage is full the wave stops with the
 // unwritten matches pending, the workgroup flushes and the round resumes.
 // ---------------------------------------------------------------------------
-template <int KB, int BLOCK, int NB, int R, bool UNIQUE>
+// DIGIT_SENTINEL: every key of partition q has low bits q (the join pipeline
+// partitions by the low bits itself), so q ^ 1 is absent from partition q and
+// marks empty slots; otherwise (caller-partitioned tuples) each item finds a
+// value absent from its build chunk.
+template <int KB, int BLOCK, int NB, int R, bool UNIQUE, bool DIGIT_SENTINEL>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void k_join(JoinArgs A) {
   using KTr = KeyT<KB>;
   using key_t = typename KTr::key_t;
@@ -268,6 +284,11 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
   uint2* s_stage = reinterpret_cast<uint2*>(s_raw + size_t(A.table_cap) * (sizeof(key_t) + 4));  // STAGE pairs
   __shared__ StageCtl ctl;
   __shared__ uint32_t s_any[3];
+  // sentinel search (caller-partitioned tuples): chunk minimum and maximum,
+  // and a bitmap of the values 1..c+1 for a chunk holding both extremes
+  constexpr int kGapWords = DIGIT_SENTINEL ? 1 : (NB * BLOCK + 2 + 31) / 32;
+  __shared__ key_t s_ext[2];
+  __shared__ uint32_t s_gap[kGapWords + 1];
   uint32_t vote = 0;
 
   const int tid = threadIdx.x;
@@ -306,6 +327,64 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
     }
   };
   if (d.valid) load_item(d, bt, pt);
+  // A value absent from the item's build chunk, for caller-partitioned
+  // tuples: the chunk's minimum - 1, else its maximum + 1, else (the chunk
+  // holds both extremes) the smallest of 1..c+1 it lacks, which exists since
+  // the chunk has c keys.  Uniform result; every branch is uniform.
+  auto chunk_sentinel = [&](const ItemDesc& x) -> key_t {
+    using ull = unsigned long long;
+    key_t mn = ~key_t(0), mx = 0;
+#pragma unroll
+    for (int k = 0; k < NB; ++k)
+      if (x.rlo + uint32_t(k) * BLOCK + tid < x.rhi) {
+        const key_t v = KTr::key(bt[k]);
+        mn = v < mn ? v : mn;
+        mx = v > mx ? v : mx;
+      }
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+      const key_t a = key_t(__shfl_xor(mn, off, kWave)), b = key_t(__shfl_xor(mx, off, kWave));
+      mn = a < mn ? a : mn;
+      mx = b > mx ? b : mx;
+    }
+    if (tid == 0) {
+      s_ext[0] = ~key_t(0);
+      s_ext[1] = 0;
+    }
+    lds_barrier();
+    if (lane_id() == 0) {
+      if constexpr (sizeof(key_t) == 8) {
+        atomicMin(reinterpret_cast<ull*>(&s_ext[0]), ull(mn));
+        atomicMax(reinterpret_cast<ull*>(&s_ext[1]), ull(mx));
+      } else {
+        atomicMin(&s_ext[0], mn);
+        atomicMax(&s_ext[1], mx);
+      }
+    }
+    lds_barrier();
+    const key_t gmn = s_ext[0], gmx = s_ext[1];
+    if (gmn != key_t(0)) return gmn - 1;
+    if (gmx != ~key_t(0)) return gmx + 1;
+    const uint32_t c = x.rhi - x.rlo, words = (c + 2u + 31u) / 32u;  // bits 0..c+1
+    for (uint32_t i = tid; i < words; i += BLOCK) s_gap[i] = 0;
+    if (tid == 0) s_gap[kGapWords] = 0xFFFFFFFFu;
+    lds_barrier();
+#pragma unroll
+    for (int k = 0; k < NB; ++k)
+      if (x.rlo + uint32_t(k) * BLOCK + tid < x.rhi) {
+        const key_t v = KTr::key(bt[k]);
+        if (v >= key_t(1) && v <= key_t(c + 1u)) atomicOr(&s_gap[uint32_t(v) / 32u], 1u << (uint32_t(v) % 32u));
+      }
+    lds_barrier();
+    for (uint32_t i = tid; i < words; i += BLOCK) {
+      uint32_t free_bits = ~s_gap[i];
+      if (i == 0) free_bits &= ~1u;                                   // 0 is present
+      const uint32_t top = c + 1u - i * 32u;                          // highest candidate bit in this word
+      if (top < 31u) free_bits &= (2u << top) - 1u;
+      if (free_bits) atomicMin(&s_gap[kGapWords], i * 32u + uint32_t(__builtin_ctz(free_bits)));
+    }
+    lds_barrier();
+    return key_t(s_gap[kGapWords]);
+  };
 
   while (d.valid) {
     // The empty asm redefines this item's tuples (copied from the previous
@@ -320,8 +399,9 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
     if (dn.valid) load_item(dn, bn, pn);
     const uint64_t item_nn = item + 2 * uint64_t(gridDim.x);
     raw_desc_load(A, item_nn, ra, rb);  // decoded at the end of this item
-    const uint32_t tsize = uint32_t(d.fd.d) * BS;
-    const key_t sentinel = key_t(d.q ^ 1u);
+    const uint32_t tsize = BS << d.lg, tmask = tsize - 1u;
+    key_t sentinel = key_t(d.q ^ 1u);
+    if constexpr (!DIGIT_SENTINEL) sentinel = chunk_sentinel(d);
     {
       const uint4 fill = bucket_fill(sentinel);
       uint4* s4 = reinterpret_cast<uint4*>(s_keys);
@@ -344,7 +424,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
       for (int k = 0; k < NB; ++k) {
         const bool v = d.rlo + uint32_t(k) * BLOCK + tid < d.rhi;
         pend |= v ? (1u << k) : 0u;
-        bslot[k] = v ? BS * hash_slot(murmur3(KTr::key(bt[k])), d.fd) : 0u;
+        bslot[k] = v ? first_slot<BS>(KTr::key(bt[k]), d.lg) : 0u;
       }
       while (pend) {
         key_t old[NB];
@@ -359,7 +439,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
               pend &= ~(1u << k);
             } else {
               dup |= old[k] == KTr::key(bt[k]);
-              bslot[k] = bslot[k] + 1 == tsize ? 0u : bslot[k] + 1;
+              bslot[k] = (bslot[k] + 1u) & tmask;
             }
           }
       }
@@ -379,9 +459,10 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
       uint32_t walk = 0, found = 0;
 #pragma unroll
       for (int k = 0; k < R; ++k) {
-        const bool v = r0 + uint32_t(k) * BLOCK + tid < d.shi;
+        bool v = r0 + uint32_t(k) * BLOCK + tid < d.shi;
+        if constexpr (!DIGIT_SENTINEL) v = v && KTr::key(ptr[k]) != sentinel;
         walk |= v ? (1u << k) : 0u;
-        slot[k] = v ? BS * hash_slot(murmur3(KTr::key(ptr[k])), d.fd) : 0u;
+        slot[k] = v ? first_slot<BS>(KTr::key(ptr[k]), d.lg) : 0u;
       }
       while (__ballot(walk != 0)) {
 #pragma unroll
@@ -405,7 +486,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
             } else if (empty) {
               walk &= ~(1u << k);
             } else {
-              slot[k] = slot[k] + BS == tsize ? 0u : slot[k] + BS;
+              slot[k] = (slot[k] + BS) & tmask;
             }
           }
       }
@@ -439,9 +520,10 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
       uint32_t walk = 0;  // bit k: probe k has not reached its end
 #pragma unroll
       for (int k = 0; k < R; ++k) {
-        const bool v = r0 + uint32_t(k) * BLOCK + tid < d.shi;
+        bool v = r0 + uint32_t(k) * BLOCK + tid < d.shi;
+        if constexpr (!DIGIT_SENTINEL) v = v && KTr::key(ptr[k]) != sentinel;
         walk |= v ? (1u << k) : 0u;
-        slot[k] = v ? BS * hash_slot(murmur3(KTr::key(ptr[k])), d.fd) : 0u;
+        slot[k] = v ? first_slot<BS>(KTr::key(ptr[k]), d.lg) : 0u;
       }
       while (true) {
         while (__ballot(walk != 0)) {
@@ -455,7 +537,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
             if (walk & (1u << k)) {
               if (e[k] == KTr::key(ptr[k])) hit |= 1u << k;
               else if (e[k] == sentinel) walk &= ~(1u << k);
-              else slot[k] = slot[k] + 1 == tsize ? 0u : slot[k] + 1;
+              else slot[k] = (slot[k] + 1u) & tmask;
             }
           if (__ballot(hit != 0)) {
             // wave prefix sum of the per-lane match counts from CNT_BITS ballots
@@ -476,7 +558,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
               if (hit & (1u << k)) {
                 if (pos < STAGE) {
                   s_stage[pos] = make_uint2(uint32_t(s_pays[slot[k]]), uint32_t(KTr::pay(ptr[k])));
-                  slot[k] = slot[k] + 1 == tsize ? 0u : slot[k] + 1;
+                  slot[k] = (slot[k] + 1u) & tmask;
                 }
                 ++pos;  // a match past the stage stays pending at its slot
               }
@@ -673,23 +755,32 @@ __global__ __launch_bounds__(256) void k_defrag_move(const DefragState* st, cons
 // ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
-template <int KB, int NB, int R>
+template <int KB, int NB, int R, bool DS>
 static void join_launch_t(const JoinArgs& a, int unique, size_t lds, hipStream_t s) {
   dim3 grid(kJoinGrid), block(kJoinBlock);
-  if (unique) hipLaunchKernelGGL((k_join<KB, kJoinBlock, NB, R, true>), grid, block, lds, s, a);
-  else hipLaunchKernelGGL((k_join<KB, kJoinBlock, NB, R, false>), grid, block, lds, s, a);
+  if (unique) hipLaunchKernelGGL((k_join<KB, kJoinBlock, NB, R, true, DS>), grid, block, lds, s, a);
+  else hipLaunchKernelGGL((k_join<KB, kJoinBlock, NB, R, false, DS>), grid, block, lds, s, a);
 }
 
-static hipError_t launch_join_kernel(const JoinArgs& A, int key_bytes, int base, int unique, hipStream_t s) {
-  const size_t lds = join_lds_bytes(key_bytes, A.table_cap);
+template <bool DS>
+static void launch_join_kernel_t(const JoinArgs& A, int key_bytes, int base, int unique, size_t lds, hipStream_t s) {
   // build tuples per thread: NB * kJoinBlock >= base (base <= table_cap / 2 - 1)
   if (key_bytes == 4) {
-    if (base <= 3 * kJoinBlock) join_launch_t<4, 3, kJoinProbe4>(A, unique, lds, s);
-    else join_launch_t<4, 6, kJoinProbe4>(A, unique, lds, s);
+    if (base <= 3 * kJoinBlock) join_launch_t<4, 3, kJoinProbe4, DS>(A, unique, lds, s);
+    else join_launch_t<4, 6, kJoinProbe4, DS>(A, unique, lds, s);
   } else {
-    if (base <= 2 * kJoinBlock) join_launch_t<8, 2, kJoinProbe8>(A, unique, lds, s);
-    else join_launch_t<8, 4, kJoinProbe8>(A, unique, lds, s);
+    if (base <= 2 * kJoinBlock) join_launch_t<8, 2, kJoinProbe8, DS>(A, unique, lds, s);
+    else join_launch_t<8, 4, kJoinProbe8, DS>(A, unique, lds, s);
   }
+}
+
+// digit_sentinel: the tuples were partitioned by their low key bits here, so
+// partition q's keys all have low bits q (see k_join).
+static hipError_t launch_join_kernel(const JoinArgs& A, int key_bytes, int base, int unique, bool digit_sentinel,
+                                     hipStream_t s) {
+  const size_t lds = join_lds_bytes(key_bytes, A.table_cap);
+  if (digit_sentinel) launch_join_kernel_t<true>(A, key_bytes, base, unique, lds, s);
+  else launch_join_kernel_t<false>(A, key_bytes, base, unique, lds, s);
   return hipGetLastError();
 }
 
@@ -715,16 +806,17 @@ static JoinArgs join_args(const Layout& L, int key_bytes, int base, int64_t slic
   A.desc_cap = L.desc_cap;
   A.table_cap = join_table_capacity(key_bytes);
   A.bucket_slots = 16 / key_bytes;
+  A.cap_buckets = 1u << (31 - __builtin_clz(uint32_t(A.table_cap / A.bucket_slots)));
   return A;
 }
 
 hipError_t launch_join(const Layout& L, int key_bytes, int base, int64_t slice, int unique, pj_pair* out,
-                       int64_t capacity, hipStream_t s) {
+                       int64_t capacity, hipStream_t s, bool digit_sentinel) {
   JoinArgs A = join_args(L, key_bytes, base, slice, out, capacity);
   hipLaunchKernelGGL(k_items_expand, dim3(1024), dim3(256), 0, s, A, reinterpret_cast<uint4*>(L.desc), L.desc_cap);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  return launch_join_kernel(A, key_bytes, base, unique, s);
+  return launch_join_kernel(A, key_bytes, base, unique, digit_sentinel, s);
 }
 
 hipError_t launch_defrag(const Layout& L, pj_pair* out, int64_t capacity, int64_t* d_count, hipStream_t s) {
@@ -783,7 +875,7 @@ hipError_t launch_join_generic(const void* rt, const uint32_t* roff, const void*
   L.desc = take(L.desc_cap * 32);
   hipError_t e = launch_join_plan(L, base, slice, s);
   if (e != hipSuccess) return e;
-  e = launch_join(L, key_bytes, base, slice, unique, out, capacity, s);
+  e = launch_join(L, key_bytes, base, slice, unique, out, capacity, s, false);
   if (e != hipSuccess) return e;
   return launch_defrag(L, out, capacity, d_count, s);
 }

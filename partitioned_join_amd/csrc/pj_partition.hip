@@ -707,10 +707,13 @@ __host__ __device__ constexpr size_t scatter_lds_bytes(int F) {
 //   8-byte keys  (16-byte tuples) F <= 64: 8K + carry (pass 2), F <= 128: 8K +
 //                carry (pass 1); F <= 256: 4K (pass 1) or 7K (pass 2) + carry;
 //                512: 4K + carry; 1024: 8K; more: 4K.
+// Pass 2 of a deep plan with 4096 x 4096 partitions: the item prefix over
+// 4096 pass-1 partitions leaves room for 7K (4-byte keys) or 3K tuples.
 struct ScatterCfg {
   int tile, cf;
 };
-inline ScatterCfg scatter_cfg(int key_bytes, int F, bool pass2) {
+inline ScatterCfg scatter_cfg(int key_bytes, int F, bool pass2, int F1 = 0) {
+  if (pass2 && F > 2048 && F1 > 2048) return {key_bytes == 4 ? 7168 : 3072, 0};
   if (key_bytes == 4) {
     if (F <= 64) return {16384, 64};
     if (F <= 256) return {pass2 ? 14336 : 12288, 256};
@@ -953,9 +956,56 @@ hipError_t launch_pass1(const Layout& L, int key_bytes, hipStream_t s) {
 }
 
 // ---------------------------------------------------------------------------
-// k_pass2: items (p1, segment group) -> split by digit key & (F2 - 1).
+// Pass-2 work items.  Pass-1 partition p1 is split into ng(p1) items, numbered
+// in XCD-major partition order r = (p1 % 8) * PX + p1 / 8, PX = ceil(F1 / 8):
+//   combined-histogram plans: ng = clamp(|p1| / chunk, 1, nseg) groups of
+//     consecutive segments (0 for an empty p1), whose per-digit output starts
+//     the [segment][digit] matrix gives;
+//   deep plans: ng = ceil(|p1| / chunk) consecutive ranges of `chunk` tuples,
+//     whose per-digit counts k_hist2 takes over pass 1's output.
+// s_pre[r] counts the items before rank r, s_pre[RN] all of them.  Ends with
+// a barrier.
 // ---------------------------------------------------------------------------
-template <int KB, int BLOCK, int TILE, int CF>
+__device__ __forceinline__ uint32_t p2_rank_to_p1(uint32_t r, uint32_t PX) { return (r % PX) * kNumXcd + r / PX; }
+
+template <int BLOCK, bool DEEP>
+__device__ __forceinline__ void pass2_item_prefix(const PartTable& T, int F1, int b2, uint32_t chunk, uint32_t* s_pre,
+                                                  uint32_t* s_tmp) {
+  const uint32_t PX = uint32_t(F1 + kNumXcd - 1) / kNumXcd, RN = PX * kNumXcd;
+  constexpr int RPT = (kMaxRadix + BLOCK - 1) / BLOCK;  // ranks per thread
+  const int tid = threadIdx.x;
+  uint32_t ng[RPT], sum = 0;
+#pragma unroll
+  for (int i = 0; i < RPT; ++i) {
+    const uint32_t r = uint32_t(tid) * RPT + i, p1 = r < RN ? p2_rank_to_p1(r, PX) : 0xFFFFFFFFu;
+    ng[i] = 0;
+    if (p1 < uint32_t(F1)) {
+      if constexpr (DEEP) {
+        const uint32_t n = T.offc[p1 + 1] - T.offc[p1];
+        ng[i] = (n + chunk - 1) / chunk;
+      } else {
+        const uint32_t n = T.off[(p1 + 1) << b2] - T.off[p1 << b2];
+        ng[i] = n == 0 ? 0u : min(T.nseg, max(1u, n / chunk));
+      }
+    }
+    sum += ng[i];
+  }
+  uint32_t total;
+  uint32_t run = block_exclusive_scan<BLOCK>(sum, s_tmp, &total);
+#pragma unroll
+  for (int i = 0; i < RPT; ++i) {
+    const uint32_t r = uint32_t(tid) * RPT + i;
+    if (r < RN) s_pre[r] = run;
+    run += ng[i];
+  }
+  if (tid == 0) s_pre[RN] = total;
+  lds_barrier();
+}
+
+// ---------------------------------------------------------------------------
+// k_pass2: items (p1, segment group or range) -> split by digit key & (F2 - 1).
+// ---------------------------------------------------------------------------
+template <int KB, int BLOCK, int TILE, int CF, bool DEEP>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void k_pass2(PartTable t0, PartTable t1,
                                                                                         PartParams pp, uint32_t g0) {
   using tup_t = typename ScatterTup<KB>::type;  // the tuple as a vector
@@ -981,42 +1031,16 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
   uint32_t* s_delta = s_cur + F2;
   uint32_t* s_start = s_delta + F2;
   tup_t* s_line = reinterpret_cast<tup_t*>(s_start + (CF > 0 ? F2 : 0));
-  // Work items: pass-1 partition p1 is split into ng(p1) groups of
-  // consecutive segments, ng = clamp(|p1| / (kPass2ItemTiles tiles), 1, nseg)
-  // (0 for an empty p1), so an item holds about kPass2ItemTiles full tiles
-  // whatever the skew.  Items are numbered in XCD-major partition order
-  // r = (p1 % 8) * PX + p1 / 8; s_pre[r] counts the items before rank r.
+  // Work items (pass2_item_prefix): an item holds about kPass2ItemTiles full
+  // tiles whatever the skew.
   const uint32_t PX = uint32_t(F1 + kNumXcd - 1) / kNumXcd, RN = PX * kNumXcd;
   uint32_t* s_pre = reinterpret_cast<uint32_t*>(s_line + CF * Carry<KB>::LT);  // RN + 1 entries
   uint32_t* s_tmp = s_pre + RN + 1;   // BLOCK / 64 + 1 entries
   const int tid = threadIdx.x;
   const uint32_t chunk = uint32_t(kPass2ItemTiles) * uint32_t(TILE);
-  auto p1_of = [&](uint32_t r) { return (r % PX) * kNumXcd + r / PX; };
-  {
-    constexpr int RPT = (kMaxRadix + BLOCK - 1) / BLOCK;  // ranks per thread
-    uint32_t ng[RPT], sum = 0;
-#pragma unroll
-    for (int i = 0; i < RPT; ++i) {
-      const uint32_t r = uint32_t(tid) * RPT + i, p1 = r < RN ? p1_of(r) : 0xFFFFFFFFu;
-      ng[i] = 0;
-      if (p1 < uint32_t(F1)) {
-        const uint32_t n = T.off[(p1 + 1) << b2] - T.off[p1 << b2];
-        ng[i] = n == 0 ? 0u : min(nseg, max(1u, n / chunk));
-      }
-      sum += ng[i];
-    }
-    uint32_t total;
-    uint32_t run = block_exclusive_scan<BLOCK>(sum, s_tmp, &total);
-#pragma unroll
-    for (int i = 0; i < RPT; ++i) {
-      const uint32_t r = uint32_t(tid) * RPT + i;
-      if (r < RN) s_pre[r] = run;
-      run += ng[i];
-    }
-    if (tid == 0) s_pre[RN] = total;
-  }
+  auto p1_of = [&](uint32_t r) { return p2_rank_to_p1(r, PX); };
   for (int d = tid; d < F2; d += BLOCK) s_cnt[d] = 0;
-  lds_barrier();
+  pass2_item_prefix<BLOCK, DEEP>(T, F1, b2, chunk, s_pre, s_tmp);
   // This workgroup's items (LDS values read as scalars: every lane holds the
   // same): the items, in rank order, are cut into 8 equal runs and the
   // workgroups of XCD x stride over run x.  With equal XCD shares run x is
@@ -1052,16 +1076,24 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
     }
     const int p1 = int(p1_of(lo));
     const uint32_t NG = pre(lo + 1) - pre(lo), g = k - pre(lo);
-    const uint32_t seg_lo = g * nseg / NG, seg_hi = (g + 1) * nseg / NG;
     const uint32_t q0 = uint32_t(p1) << b2;
-    const uint32_t base1 = T.off[q0];
-    const uint32_t in_lo = base1 + T.matc[size_t(seg_lo) * F1 + p1];
-    const uint32_t in_hi = seg_hi < nseg ? base1 + T.matc[size_t(seg_hi) * F1 + p1] : T.off[q0 + F2];
+    uint32_t in_lo, in_hi, seg_lo = 0;
+    if constexpr (DEEP) {
+      in_lo = T.offc[p1] + g * chunk;
+      in_hi = min(in_lo + chunk, T.offc[p1 + 1]);
+    } else {
+      seg_lo = g * nseg / NG;
+      const uint32_t seg_hi = (g + 1) * nseg / NG;
+      const uint32_t base1 = T.off[q0];
+      in_lo = base1 + T.matc[size_t(seg_lo) * F1 + p1];
+      in_hi = seg_hi < nseg ? base1 + T.matc[size_t(seg_hi) * F1 + p1] : T.off[q0 + F2];
+    }
     if (in_lo >= in_hi || in_hi > T.n) continue;
     // s_cur and s_start were last read by the previous tile's scan, before
     // its staging barrier (the flush after a range's last tile reads neither)
     for (int d = tid; d < F2; d += BLOCK) {
-      const uint32_t c0 = T.off[q0 + d] + T.mat[size_t(seg_lo) * P + q0 + d];
+      const uint32_t c0 = DEEP ? T.off[q0 + d] + T.h2[size_t(k) * F2 + d]
+                               : T.off[q0 + d] + T.mat[size_t(seg_lo) * P + q0 + d];
       s_cur[d] = c0;
       if (CF > 0) s_start[d] = c0;
     }
@@ -1128,9 +1160,10 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
   }
 }
 
-hipError_t launch_pass2(const Layout& L, int key_bytes, hipStream_t s) {
-  // Persistent workgroups, kPass2WgPerXcd per XCD and table on average, split
-  // between the tables by their sizes (the work is proportional to them).
+// Persistent pass-2 workgroups, kPass2WgPerXcd per XCD and table on average,
+// split between the tables by their sizes (the work is proportional to them):
+// the grid, and g0, the first workgroup of table 1 (a multiple of 8).
+static void pass2_grid(const Layout& L, uint32_t* grid, uint32_t* g0) {
   const uint32_t per_xcd = uint32_t(kPass2WgPerXcd) * uint32_t(L.ntables);
   uint32_t x0 = per_xcd;
   if (L.ntables > 1) {
@@ -1140,16 +1173,34 @@ hipError_t launch_pass2(const Layout& L, int key_bytes, hipStream_t s) {
     if (x0 < 1) x0 = 1;
     if (x0 > per_xcd - 1) x0 = per_xcd - 1;
   }
-  const uint32_t g0 = x0 * uint32_t(kNumXcd);
-  dim3 grid(per_xcd * kNumXcd), block(kScatterBlock);
+  *g0 = x0 * uint32_t(kNumXcd);
+  *grid = per_xcd * kNumXcd;
+}
+
+static uint32_t pass2_chunk(int key_bytes, const PartParams& pp) {
+  return uint32_t(kPass2ItemTiles) * uint32_t(scatter_cfg(key_bytes, pp.F2, true, pp.F1).tile);
+}
+
+uint64_t deep_item_capacity(int64_t n, int key_bytes, const PartParams& pp) {
+  return uint64_t(n) / pass2_chunk(key_bytes, pp) + uint64_t(pp.F1) + 1;
+}
+
+hipError_t launch_pass2(const Layout& L, int key_bytes, hipStream_t s) {
+  uint32_t ngrid = 0, g0 = 0;
+  pass2_grid(L, &ngrid, &g0);
+  const bool deep = L.pp.B > kMaxCombinedBits;
+  dim3 grid(ngrid), block(kScatterBlock);
   // the item prefix (one entry per pass-1 partition rounded up to 8, plus one)
   // and the scan's temporaries follow the scatter's LDS
   const size_t extra = (size_t((L.pp.F1 + kNumXcd - 1) / kNumXcd) * kNumXcd + 1 + kScatterBlock / kWave + 1) * 4;
-  const ScatterCfg c = scatter_cfg(key_bytes, L.pp.F2, true);
+  const ScatterCfg c = scatter_cfg(key_bytes, L.pp.F2, true, L.pp.F1);
 #define PJ_PASS2(KB_, TILE_, CF_)                                                                                   \
   if (c.tile == TILE_ && c.cf == CF_) {                                                                             \
-    hipLaunchKernelGGL((k_pass2<KB_, kScatterBlock, TILE_, CF_>), grid, block,                                     \
-                       (scatter_lds_bytes<KB_, TILE_, CF_>(L.pp.F2) + extra), s, L.t[0], L.t[1], L.pp, g0);        \
+    const size_t lds_ = scatter_lds_bytes<KB_, TILE_, CF_>(L.pp.F2) + extra;                                        \
+    if (deep) hipLaunchKernelGGL((k_pass2<KB_, kScatterBlock, TILE_, CF_, true>), grid, block, lds_, s, L.t[0],    \
+                                 L.t[1], L.pp, g0);                                                                 \
+    else hipLaunchKernelGGL((k_pass2<KB_, kScatterBlock, TILE_, CF_, false>), grid, block, lds_, s, L.t[0],        \
+                            L.t[1], L.pp, g0);                                                                      \
     return hipGetLastError();                                                                                       \
   }
   if (key_bytes == 4) {
@@ -1158,15 +1209,101 @@ hipError_t launch_pass2(const Layout& L, int key_bytes, hipStream_t s) {
     PJ_PASS2(4, 8192, 512)
     PJ_PASS2(4, 16384, 0)
     PJ_PASS2(4, 8192, 0)
+    PJ_PASS2(4, 7168, 0)
   } else {
     PJ_PASS2(8, 8192, 64)
     PJ_PASS2(8, 7168, 256)
     PJ_PASS2(8, 4096, 512)
     PJ_PASS2(8, 8192, 0)
     PJ_PASS2(8, 4096, 0)
+    PJ_PASS2(8, 3072, 0)
   }
 #undef PJ_PASS2
   return hipErrorInvalidValue;
+}
+
+// ---------------------------------------------------------------------------
+// Deep plans (more partitions than the combined histogram holds): after
+// pass 1, k_hist2 counts the pass-2 digits of every pass-2 item over pass 1's
+// output (the reference's histogram_pass_two, src/partition_kernels.cuh:328-406,
+// per item rather than per block), k_segscan turns the counts into each
+// item's per-digit output starts within its pass-1 partition and the fine
+// partition totals, and the offsets scan gives the final partition starts.
+// ---------------------------------------------------------------------------
+template <int KB, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_hist2(PartTable t0, PartTable t1, PartParams pp, uint32_t chunk,
+                                                 uint32_t g0) {
+  using tup_t = typename KeyT<KB>::tup_t;
+  const bool second = blockIdx.x >= g0;
+  const PartTable T = second ? t1 : t0;
+  const uint32_t local = second ? blockIdx.x - g0 : blockIdx.x;
+  const uint32_t groups = second ? gridDim.x - g0 : g0;
+  const int F1 = pp.F1, F2 = pp.F2;
+  const uint32_t mask = uint32_t(F2 - 1);
+  const uint32_t PX = uint32_t(F1 + kNumXcd - 1) / kNumXcd, RN = PX * kNumXcd;
+  extern __shared__ uint32_t s_mem[];
+  uint32_t* s_bins = s_mem;            // F2
+  uint32_t* s_pre = s_bins + F2;       // RN + 1
+  uint32_t* s_tmp = s_pre + RN + 1;    // BLOCK / 64 + 1
+  const int tid = threadIdx.x;
+  pass2_item_prefix<BLOCK, true>(T, F1, pp.b2, chunk, s_pre, s_tmp);
+  if (local == 0)
+    for (uint32_t r = tid; r <= RN; r += BLOCK) T.ipre[r] = s_pre[r];
+  const uint32_t n_items = s_pre[RN];
+  const tup_t* in = static_cast<const tup_t*>(T.buf1);
+  for (uint32_t k = local; k < n_items; k += groups) {
+    uint32_t lo = 0, hi = RN;  // s_pre[lo] <= k < s_pre[hi]
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (s_pre[mid] <= k) lo = mid; else hi = mid;
+    }
+    const uint32_t p1 = p2_rank_to_p1(lo, PX), g = k - s_pre[lo];
+    const uint32_t in_lo = T.offc[p1] + g * chunk, in_hi = min(in_lo + chunk, T.offc[p1 + 1]);
+    for (int d = tid; d < F2; d += BLOCK) s_bins[d] = 0;
+    lds_barrier();
+    for (uint32_t i = in_lo + tid; i < in_hi; i += BLOCK) atomicAdd(&s_bins[in[i].x & mask], 1u);
+    lds_barrier();
+    for (int d = tid; d < F2; d += BLOCK) T.h2[size_t(k) * F2 + d] = s_bins[d];
+    lds_barrier();
+  }
+}
+
+hipError_t launch_hist2(const Layout& L, int key_bytes, hipStream_t s) {
+  uint32_t ngrid = 0, g0 = 0;
+  pass2_grid(L, &ngrid, &g0);
+  const uint32_t RN = uint32_t((L.pp.F1 + kNumXcd - 1) / kNumXcd) * kNumXcd;
+  const size_t lds = (size_t(L.pp.F2) + RN + 1 + kScatterBlock / kWave + 1) * 4;
+  const uint32_t chunk = pass2_chunk(key_bytes, L.pp);
+  if (key_bytes == 4)
+    hipLaunchKernelGGL((k_hist2<4, kScatterBlock>), dim3(ngrid), dim3(kScatterBlock), lds, s, L.t[0], L.t[1], L.pp,
+                       chunk, g0);
+  else
+    hipLaunchKernelGGL((k_hist2<8, kScatterBlock>), dim3(ngrid), dim3(kScatterBlock), lds, s, L.t[0], L.t[1], L.pp,
+                       chunk, g0);
+  return hipGetLastError();
+}
+
+// One thread per fine partition q = p1 * F2 + d: the exclusive scan of digit
+// d's counts over p1's items (in place) and their total, tot[q].
+__global__ __launch_bounds__(256) void k_segscan(PartTable t0, PartTable t1, PartParams pp) {
+  const PartTable T = blockIdx.y ? t1 : t0;
+  const uint32_t q = blockIdx.x * 256u + threadIdx.x;
+  if (q >= uint32_t(pp.P)) return;
+  const uint32_t F2 = uint32_t(pp.F2), p1 = q >> pp.b2, d = q & (F2 - 1u);
+  const uint32_t PX = uint32_t(pp.F1 + kNumXcd - 1) / kNumXcd;
+  const uint32_t r = (p1 % kNumXcd) * PX + p1 / kNumXcd;
+  uint32_t acc = 0;
+  for (uint32_t k = T.ipre[r]; k < T.ipre[r + 1]; ++k) {
+    const uint32_t v = T.h2[size_t(k) * F2 + d];
+    T.h2[size_t(k) * F2 + d] = acc;
+    acc += v;
+  }
+  T.tot[q] = acc;
+}
+
+hipError_t launch_segscan(const Layout& L, hipStream_t s) {
+  hipLaunchKernelGGL(k_segscan, dim3((L.pp.P + 255) / 256, L.ntables), dim3(256), 0, s, L.t[0], L.t[1], L.pp);
+  return hipGetLastError();
 }
 
 }  // namespace pj

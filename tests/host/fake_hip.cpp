@@ -32,7 +32,12 @@ struct Alloc {
   uintptr_t lo, hi;
   std::string name;
 };
-std::vector<Alloc> g_allocs;
+// Registered allocations; __device__ variables are registered by the objects'
+// constructors, which may run before this file's, hence a function-local static.
+std::vector<Alloc>& allocs() {
+  static std::vector<Alloc> v;
+  return v;
+}
 int g_errors = 0;
 long g_launches = 0;
 // Kernel names by host stub; filled by the objects' constructors, which may
@@ -56,7 +61,7 @@ void err(const char* fmt, ...) {
 void need(const char* kern, const char* what, const void* p, uint64_t bytes) {
   if (bytes == 0) return;
   const uintptr_t a = reinterpret_cast<uintptr_t>(p);
-  for (const Alloc& al : g_allocs)
+  for (const Alloc& al : allocs())
     if (a >= al.lo && a < al.hi) {
       if (a + bytes > al.hi)
         err("%s: %s [%p, +%llu) overruns %s by %llu bytes", kern, what, p, (unsigned long long)bytes,
@@ -130,10 +135,15 @@ using namespace pj;
 
 void check_table(const char* k, const PartTable& T, const PartParams& pp, int kb, bool scatter1, bool scatter2) {
   const uint64_t tb = kb == 4 ? 8 : 16;
+  const bool deep = pp.passes == 2 && pp.B > kMaxCombinedBits;
   if (scatter1 || !scatter2) need(k, "keys", T.keys, uint64_t(T.n) * kb);
   if (scatter1 && T.pay) need(k, "payload", T.pay, uint64_t(T.n) * 4);
-  need(k, "mat", T.mat, uint64_t(T.nseg) * pp.P * 4);
-  if (pp.passes == 2) need(k, "matc", T.matc, uint64_t(T.nseg) * pp.F1 * 4);
+  need(k, "mat", T.mat, uint64_t(T.nseg) * (deep ? pp.F1 : pp.P) * 4);
+  if (pp.passes == 2 && !deep) need(k, "matc", T.matc, uint64_t(T.nseg) * pp.F1 * 4);
+  if (deep) {
+    need(k, "offc", T.offc, uint64_t(pp.F1 + 1) * 4);
+    need(k, "h2", T.h2, deep_item_capacity(T.n, kb, pp) * pp.F2 * 4);
+  }
   if (scatter1 || scatter2) {
     need(k, "off", T.off, uint64_t(pp.P + 1) * 4);
     need(k, "buf1", T.buf1, uint64_t(T.n) * tb);
@@ -156,6 +166,10 @@ void check_join_args(const char* k, const JoinArgs& A, uint32_t grid) {
   need(k, "wg_last", A.wg_last, uint64_t(grid) * 8);
   need(k, "desc", A.desc, A.desc_cap * 32);
   if (A.base < 1 || A.base > A.table_cap / 2 - 1) err("%s: base %d outside [1, %d]", k, A.base, A.table_cap / 2 - 1);
+  // power-of-two bucket tables: within the LDS slots, and more slots than the largest chunk
+  const uint32_t cb = A.cap_buckets, bs = uint32_t(A.bucket_slots);
+  if (cb == 0 || (cb & (cb - 1)) != 0 || cb * bs > uint32_t(A.table_cap) || cb * bs <= uint32_t(A.base))
+    err("%s: cap_buckets %u (x %u slots) against table_cap %d and base %d", k, cb, bs, A.table_cap, A.base);
   if (A.slice < 1) err("%s: probe slice %lld < 1", k, (long long)A.slice);
 }
 
@@ -192,6 +206,24 @@ void check_launch(const Launch& L) {
       check_table(k, t[y], pp, tmpl_int(name, 0), !two, two);
       if (!two && t[y].nseg > L.grid.x) err("%s: grid %u < nseg %u", k, L.grid.x, t[y].nseg);
     }
+  } else if (starts(name, "pj::k_hist2<")) {
+    const PartTable* t = static_cast<const PartTable*>(a0);
+    const PartParams& pp = *reinterpret_cast<const PartParams*>(reinterpret_cast<const char*>(a0) + 2 * sizeof(PartTable));
+    const int kb = tmpl_int(name, 0);
+    for (int y = 0; y < 2; ++y) {
+      if (t[y].n == 0) continue;
+      need(k, "buf1", t[y].buf1, uint64_t(t[y].n) * (kb == 4 ? 8 : 16));
+      need(k, "offc", t[y].offc, uint64_t(pp.F1 + 1) * 4);
+      need(k, "h2", t[y].h2, deep_item_capacity(t[y].n, kb, pp) * pp.F2 * 4);
+      need(k, "ipre", t[y].ipre, uint64_t((pp.F1 + 7) / 8 * 8 + 1) * 4);
+    }
+  } else if (starts(name, "pj::k_segscan")) {
+    const PartTable* t = static_cast<const PartTable*>(a0);
+    const PartParams& pp = *reinterpret_cast<const PartParams*>(reinterpret_cast<const char*>(a0) + 2 * sizeof(PartTable));
+    for (uint32_t y = 0; y < L.grid.y; ++y) {
+      need(k, "tot", t[y].tot, uint64_t(pp.P) * 4);
+      need(k, "ipre", t[y].ipre, uint64_t((pp.F1 + 7) / 8 * 8 + 1) * 4);
+    }
   } else if (starts(name, "pj::k_items_expand")) {
     const JoinArgs& A = *static_cast<const JoinArgs*>(a0);
     need(k, "item_scan", A.item_scan, uint64_t(A.P + 1) * 8);
@@ -208,12 +240,13 @@ extern "C" {
 
 // ---- driver-side hooks ----
 void fake_register(const void* p, size_t bytes, const char* name) {
-  g_allocs.push_back({reinterpret_cast<uintptr_t>(p), reinterpret_cast<uintptr_t>(p) + bytes, name});
+  allocs().push_back({reinterpret_cast<uintptr_t>(p), reinterpret_cast<uintptr_t>(p) + bytes, name});
 }
 void fake_unregister(const void* p) {
-  for (size_t i = 0; i < g_allocs.size(); ++i)
-    if (g_allocs[i].lo == reinterpret_cast<uintptr_t>(p)) {
-      g_allocs.erase(g_allocs.begin() + long(i));
+  std::vector<Alloc>& v = allocs();
+  for (size_t i = 0; i < v.size(); ++i)
+    if (v[i].lo == reinterpret_cast<uintptr_t>(p)) {
+      v.erase(v.begin() + long(i));
       return;
     }
 }
@@ -230,7 +263,9 @@ void __hipRegisterFunction(void**, const void* host_fn, char*, const char* devic
                            dim3*, dim3*, int*) {
   funcs()[host_fn] = demangle(device_name);
 }
-void __hipRegisterVar(void**, void*, char*, char*, int, size_t, int, int) {}
+void __hipRegisterVar(void**, void* var, char*, char* name, int, size_t size, int, int) {
+  allocs().push_back({reinterpret_cast<uintptr_t>(var), reinterpret_cast<uintptr_t>(var) + size, name});
+}
 
 hipError_t __hipPushCallConfiguration(dim3 grid, dim3 block, size_t shmem, hipStream_t stream) {
   g_cfg.push_back({grid, block, shmem, stream});
@@ -254,7 +289,8 @@ hipError_t hipLaunchKernel(const void* fn, dim3 grid, dim3 block, void** args, s
   auto it = funcs().find(fn);
   const std::string name = it == funcs().end() ? std::string("?") : it->second;
   size_t first = 8;
-  if (starts(name, "pj::k_hist<") || starts(name, "pj::k_pass1<") || starts(name, "pj::k_pass2<"))
+  if (starts(name, "pj::k_hist<") || starts(name, "pj::k_pass1<") || starts(name, "pj::k_pass2<") ||
+      starts(name, "pj::k_hist2<") || starts(name, "pj::k_segscan"))
     first = 2 * sizeof(PartTable) + sizeof(PartParams);  // t0, t1, pp: separate arguments, copied one by one
   else if (starts(name, "pj::k_items_expand") || starts(name, "pj::k_join<"))
     first = sizeof(JoinArgs);
@@ -274,6 +310,17 @@ hipError_t hipLaunchKernel(const void* fn, dim3 grid, dim3 block, void** args, s
 }
 
 hipError_t hipGetLastError(void) { return hipSuccess; }
+hipError_t hipGetDevice(int* d) {
+  *d = 0;
+  return hipSuccess;
+}
+// A __device__ variable's host-side stand-in is the "device" copy here; it is
+// registered as an allocation of its size (given by the driver via
+// fake_symbol_size, since the registration call is not informative here).
+hipError_t hipGetSymbolAddress(void** p, const void* symbol) {
+  *p = const_cast<void*>(symbol);
+  return hipSuccess;
+}
 const char* hipGetErrorString(hipError_t) { return "fake_hip error"; }
 
 hipError_t hipStreamCreateWithFlags(hipStream_t* s, unsigned int) {

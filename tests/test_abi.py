@@ -67,7 +67,7 @@ def _tables(lib_mod, nb=100, npr=100, kb=4, kb2=None):
 
 
 @pytest.mark.parametrize("case,status", [
-    ("bad_key_bytes", 1), ("negative_n", 1), ("mixed_widths", 1), ("bad_passes", 1), ("too_many_bits", 5),
+    ("bad_key_bytes", 1), ("negative_n", 1), ("mixed_widths", 1), ("bad_passes", 1), ("pass_bits_over_12", 1),
     ("bits2_one_pass", 1), ("small_workspace", 3), ("null_out", 1), ("bad_factor", 1)])
 def test_argument_errors_without_device_work(pj, case, status):
     from partitioned_join_amd import _lib
@@ -83,8 +83,8 @@ def test_argument_errors_without_device_work(pj, case, status):
         p.key_bytes = 8
     elif case == "bad_passes":
         plan = _lib.PjPlan(3, 4, 4, 0, 3.0, 0, 2)
-    elif case == "too_many_bits":
-        plan = _lib.PjPlan(2, 12, 12, 0, 3.0, 0, 2)
+    elif case == "pass_bits_over_12":
+        plan = _lib.PjPlan(2, 13, 4, 0, 3.0, 0, 2)
     elif case == "bits2_one_pass":
         plan = _lib.PjPlan(1, 4, 4, 0, 3.0, 0, 2)
     elif case == "small_workspace":

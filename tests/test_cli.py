@@ -114,3 +114,16 @@ def test_tpch_q4_tables(tmp_path):
     r = _run(["-C", "-F", "-B", str(tmp_path / "lineitem.tbl"), "-P", str(tmp_path / "missing.tbl"), "-b", "10",
               "-p", "10"])
     assert r.returncode == 1 and "cannot open" in r.stderr
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode,z", [(1, "0.25"), (2, "0.5"), (3, "0.75"), (2, "1"), (3, "1")])
+def test_skewed_data_script_flags(mode, z):
+    """The flags of every line of the reference's scripts/bench_skewed_data.sh
+    (-R -n 512 -N 512 -C -z Z -Z MODE, i.e. 2^18 partitions in two passes)
+    with the table sizes scaled from 32M to 1M: the driver's CPU check reports
+    zero errors."""
+    f = _csv(["-b", "1000000", "-p", "1000000", "-r", "2000000", "-R", "-n", "512", "-N", "512", "-z", z, "-Z",
+              str(mode)])
+    assert f[6] == "512" and f[7] == "512", f
+    assert f[-1] == "0", f

@@ -194,6 +194,25 @@ def test_int64_payload_columns_and_duplicates(pj, oracle, torch, plan):
     check_join(pj, oracle, torch, bk, pk, bp, pp, plan=plan, uniq=False)
 
 
+@pytest.mark.parametrize("plan", [{"passes": 2, "bits1": 9, "bits2": 9},
+                                  {"passes": 2, "bits1": 12, "bits2": 5, "unique_build_keys": 1},
+                                  {"passes": 2, "bits1": 6, "bits2": 12},
+                                  {"passes": 2, "bits1": 12, "bits2": 12, "max_partition_size_factor": 1.0}])
+def test_deep_plans_beyond_the_combined_histogram(pj, oracle, torch, plan):
+    """More than 2^16 partitions (the reference's skew script uses 512 x 512):
+    pass-2 digits are counted per item after pass 1.  Unique and duplicate
+    build keys, both key widths, against the oracle."""
+    rng = np.random.default_rng(plan["bits1"] * 100 + plan["bits2"])
+    for dt in (np.int32, np.int64):
+        bk = rng.permutation(1 << 20).astype(dt) if plan.get("unique_build_keys") else \
+            rng.integers(0, 1 << 19, 1 << 20).astype(dt)
+        pk = rng.integers(0, 1 << 21, 3 << 20).astype(dt)
+        if dt == np.int64:
+            bk = bk * (2**32 + 3)
+            pk = pk * (2**32 + 3)
+        check_join(pj, oracle, torch, bk, pk, plan=plan, uniq=bool(plan.get("unique_build_keys")))
+
+
 def test_capacity_error_reports_count(pj, torch):
     from partitioned_join_amd import PjError
     bk = torch.arange(1000, dtype=torch.int32, device="cuda")
@@ -230,6 +249,39 @@ def test_config2_full_size_properties(pj, torch):
     seen = torch.zeros(pk.numel(), dtype=torch.int32, device="cuda")
     seen.index_add_(0, p, torch.ones_like(pairs[:, 1]))
     assert bool((seen == match.int()).all().item())
+
+
+@pytest.mark.parametrize("theta", [0.5, 1.0, 1.5])
+def test_config3_full_size_properties(pj, torch, theta):
+    """16M x 256M Zipf(theta) at full size, checked on the device: every probe
+    key is a build key, so the count is exactly 2^28, every probe row appears
+    exactly once, and every pair joins equal keys (heavy keys split across many
+    work items on both sides)."""
+    bk, pk = pj.generators.workload_config3(theta)
+    print(f"theta {theta}: 16M x 256M", flush=True)
+    pairs, n = pj.partitioned_join(bk, pk, return_count=True)
+    assert n == 1 << 28
+    b = pairs[:, 0].long()
+    p = pairs[:, 1].long()
+    assert bool((bk[b] == pk[p]).all().item())
+    seen = torch.zeros(pk.numel(), dtype=torch.int32, device="cuda")
+    seen.index_add_(0, p, torch.ones_like(pairs[:, 1]))
+    assert bool((seen == 1).all().item())
+    del pairs, b, p, seen
+
+
+def test_output_beyond_2_to_32_pairs_reports_exact_count(pj, torch):
+    """65567 equal keys on both sides join to 65567^2 = 4,299,031,489 pairs,
+    beyond 2^32 (cuDF HashJoinLargeOutputSize, cudf:tests/join/join_tests.cpp):
+    with a small output the call returns PJ_ERR_CAPACITY and the exact 64-bit
+    count, for 4- and 8-byte keys."""
+    n = 65567
+    for dt in (torch.int32, torch.int64):
+        bk = torch.full((n,), 7, dtype=dt, device="cuda")
+        pk = torch.full((n,), 7, dtype=dt, device="cuda")
+        with pytest.raises(pj.PjError) as e:
+            pj.partitioned_join(bk, pk, out_capacity=1 << 20)
+        assert e.value.status == 2 and e.value.count == n * n, (dt, e.value.count)
 
 
 def test_gather_columns(pj, oracle, torch):

@@ -196,6 +196,62 @@ def test_assign_blocks(pj, oracle, torch):
         assert u32(pidx)[:e_nblk].tolist() == e_pidx
 
 
+@pytest.mark.parametrize("dtype", [np.int32, np.int64])
+def test_join_of_partitions_not_split_by_low_bits(pj, oracle, torch, dtype):
+    """pj_join_partitions over tuples partitioned on bits 4..7 (shift 4) and
+    joined as 16 partitions: partition q then holds keys whose low bits are
+    not q, including the value q ^ 1 (key 1 lies in partition 0).  Every key
+    value must still join, duplicates included."""
+    rng = np.random.default_rng(31)
+    bk = rng.integers(-40, 300, 20000).astype(dtype)
+    pk = rng.integers(-40, 300, 70000).astype(dtype)
+    bk[:6] = [0, 1, 2, 3, -1, 17]
+    pk[:6] = [1, 0, 3, 2, -1, 16]
+    d = lambda a: torch.from_numpy(a).cuda()
+    kb = bk.itemsize
+    tb, ob = pj.partition_input_tables_pass_one(d(bk), None, bits=4, shift=4)
+    tp, op = pj.partition_input_tables_pass_one(d(pk), None, bits=4, shift=4)
+    exp = oracle.sort_pairs(oracle.inner_join_cpu(bk, pk, uniq_keys=False))
+    for base in (1000, 2783 if kb == 4 else 2047):
+        pairs, n = pj.shared_memory_hash_join(tb, ob, tp, op, kb, 4, base, out_capacity=len(exp) + 64)
+        assert n == len(exp), (base, n, len(exp))
+        assert np.array_equal(oracle.sort_pairs(pairs.cpu().numpy()), exp), base
+
+
+@pytest.mark.parametrize("dtype", [np.int32, np.int64])
+def test_join_of_a_chunk_holding_both_extreme_keys(pj, oracle, torch, dtype):
+    """One caller-made partition whose build chunk holds the smallest and the
+    largest key value (as unsigned: 0 and all ones) and 1, 2, 3, ...: no value
+    below or above the chunk is free, so the empty marker is the smallest
+    missing value of 1..c+1."""
+    info = np.iinfo(dtype)
+    nb = 1500
+    bk = np.concatenate([np.array([0, -1, info.min, info.max], dtype=dtype), np.arange(1, nb - 3, dtype=dtype)])
+    bk = np.delete(bk, 700)  # leave a gap at 697 (the marker must be a value absent from the chunk)
+    pk = np.concatenate([bk, np.array([697, 5000, 0, -1, info.max], dtype=dtype)])
+    rng = np.random.default_rng(5)
+    bk, pk = rng.permutation(bk), rng.permutation(pk)
+    kb = bk.itemsize
+    cols = 2 if kb == 4 else 4
+
+    def tuples(keys):
+        t = np.zeros((len(keys), cols), dtype=np.int32)
+        if kb == 4:
+            t[:, 0] = keys
+        else:
+            t[:, 0:2] = keys.astype(np.int64).view(np.int32).reshape(-1, 2)
+        t[:, cols - 2 if kb == 8 else 1] = np.arange(len(keys), dtype=np.int32)
+        return torch.from_numpy(t).cuda()
+
+    # total_bits 1: partition 0 holds every tuple, partition 1 none
+    ob = torch.tensor([0, len(bk), len(bk)], dtype=torch.int32, device="cuda")
+    op = torch.tensor([0, len(pk), len(pk)], dtype=torch.int32, device="cuda")
+    exp = oracle.sort_pairs(oracle.inner_join_cpu(bk, pk, uniq_keys=False))
+    pairs, n = pj.shared_memory_hash_join(tuples(bk), ob, tuples(pk), op, kb, 1, 2047, out_capacity=len(exp) + 8)
+    assert n == len(exp)
+    assert np.array_equal(oracle.sort_pairs(pairs.cpu().numpy()), exp)
+
+
 def test_shared_memory_hash_join_phase(pj, oracle, torch):
     rng = np.random.default_rng(13)
     nb, npr, B = 200000, 600000, 10

@@ -296,3 +296,26 @@ def test_plan_learned_agreement_fallback_and_same_pairs(pj, oracle, torch):
     a = oracle.sort_pairs(pj.partitioned_join(bk, pz, plan=plan2).cpu().numpy())
     b = oracle.sort_pairs(pj.partitioned_join(bk, pz, plan=plain).cpu().numpy())
     assert len(a) == pz.numel() and np.array_equal(a, b)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("i", [2, 3, 4, 5, 6, 10, 11])
+def test_trained_weights_plan_joins_equal_to_oracle(pj, oracle, torch, i):
+    """The committed trained weights plan training-distribution joins (key
+    width, Zipf theta and match rate of planner.join_spec(i), sizes scaled to at
+    most 1M x 4M for the oracle); whatever plan the model and its fallback pick,
+    the pairs equal the oracle's join."""
+    from partitioned_join_amd import planner
+    w = planner.load_weights()
+    assert w is not None, "trained weights are committed with the package"
+    spec = planner.join_spec(i)
+    scale = max(1.0, spec["n_build"] / (1 << 20), spec["n_probe"] / (1 << 22))
+    spec = dict(spec, n_build=max(1, int(spec["n_build"] / scale)), n_probe=max(1, int(spec["n_probe"] / scale)))
+    bk, pk = planner.make_join(spec)
+    plan, fallback = planner.plan_learned(bk, pk, w)
+    print(f"join {i}: {spec['n_build']} x {spec['n_probe']}, {spec['key_bytes']}-byte keys, theta "
+          f"{spec['theta']:.2f}: plan {plan.as_dict()}, fallback {fallback}", flush=True)
+    assert plan.source == (0 if fallback else 1)
+    got = pj.partitioned_join(bk, pk, plan=plan, out_capacity=pk.numel() + 1)
+    exp = oracle.inner_join_cpu(bk.cpu().numpy(), pk.cpu().numpy())
+    assert np.array_equal(oracle.sort_pairs(got.cpu().numpy()), oracle.sort_pairs(exp))
