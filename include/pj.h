@@ -139,6 +139,18 @@ pj_status pj_partitioned_join_async(const pj_table* build, const pj_table* probe
                                     pj_pair* out, int64_t out_capacity, int64_t* d_out_count,
                                     void* workspace, size_t workspace_bytes, hipStream_t stream);
 
+/* Allocation-free form of pj_partitioned_join_async.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The workspace is a static device array of 192 MiB inside libpj.so, one per
+ * GPU, held until the process ends; joins whose pj_workspace_size exceeds it
+ * return PJ_ERR_WORKSPACE.  Nothing is allocated or freed.  Every call on a
+ * GPU shares that GPU's array, so calls that overlap on one GPU (other
+ * streams, other threads) race: serialise them. */
+pj_status pj_partitioned_join_noalloc(const pj_table* build, const pj_table* probe, const pj_plan* plan,
+                                      pj_pair* out, int64_t out_capacity, int64_t* d_out_count,
+                                      hipStream_t stream);
+
 /* Projection to the reference's output form {x = build key, y = probe payload}
  * (src/hash_join_kernels.cuh:155-164): out_xy[2i] = key of build row
  * pairs[i].build_idx, out_xy[2i+1] = pairs[i].probe_idx.  Requires pairs made

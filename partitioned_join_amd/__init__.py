@@ -170,6 +170,20 @@ def partitioned_join_async(build_keys, probe_keys, out, build_payload=None, prob
     return d_count
 
 
+def partitioned_join_noalloc(build_keys, probe_keys, out, d_count, build_payload=None, probe_payload=None, plan=None,
+                             stream=None) -> None:
+    """pj_partitioned_join_noalloc: the join into ``out`` ((cap, 2) int32) with
+    the pair count written to ``d_count`` (device int64, one element), using the
+    library's static device scratch instead of a workspace; allocates nothing.
+    Calls that overlap on one GPU race (they share the scratch)."""
+    bt = _table(build_keys, build_payload, "build")
+    pt = _table(probe_keys, probe_payload, "probe")
+    p = _plan_struct(plan)
+    _lib.check(lib().pj_partitioned_join_noalloc(ctypes.byref(bt), ctypes.byref(pt),
+                                                 ctypes.byref(p) if p is not None else None, _ptr(out), out.shape[0],
+                                                 _ptr(d_count), _stream(stream)))
+
+
 def project_reference(build_keys, pairs, stream=None):
     """(build key, probe payload) pairs: the reference's output form."""
     torch = _torch()

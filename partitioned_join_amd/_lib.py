@@ -57,6 +57,7 @@ SIGNATURES = {
     "pj_partitioned_join": (c_i32, [P_TABLE, P_TABLE, P_PLAN, c_vp, c_i64, ctypes.POINTER(c_i64), c_vp, c_size,
                                     c_vp]),
     "pj_partitioned_join_async": (c_i32, [P_TABLE, P_TABLE, P_PLAN, c_vp, c_i64, c_vp, c_vp, c_size, c_vp]),
+    "pj_partitioned_join_noalloc": (c_i32, [P_TABLE, P_TABLE, P_PLAN, c_vp, c_i64, c_vp, c_vp]),
     "pj_project_reference": (c_i32, [P_TABLE, c_vp, c_i64, c_vp, c_vp]),
     "pj_gather": (c_i32, [c_vp, c_i64, c_vp, c_i32, c_vp, c_i32, c_vp, c_vp, c_vp]),
     "pj_combined_histogram": (c_i32, [P_TABLE, c_i32, c_i32, c_vp, c_vp, c_vp, c_size, c_vp]),

@@ -10,6 +10,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -382,6 +383,59 @@ pj_status pj_partitioned_join(const pj_table* build, const pj_table* probe, cons
     return fail(PJ_ERR_CAPACITY, "join produced %lld pairs, output capacity is %lld", (long long)h,
                 (long long)out_capacity);
   return PJ_OK;
+}
+
+// ---------------------------------------------------------------------------
+// The allocation-free join: the workspace is a file-scope device array, one
+// copy per GPU, held by the loaded code object until the process ends.
+// ---------------------------------------------------------------------------
+}  // extern "C"
+
+namespace pj {
+
+constexpr size_t kNoAllocScratchBytes = size_t(192) << 20;
+static __device__ __align__(256) uint8_t scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of the current device's copy of `scratch` (the array's
+// name in host code is only a host-side stand-in), looked up once per device.
+static pj_status scratch_address(void** ptr) {
+  int dev = -1;
+  PJ_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail(PJ_ERR_UNSUPPORTED, "device index %d beyond %d", dev, kMaxDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    PJ_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p || reinterpret_cast<uintptr_t>(p) % 256 != 0)
+      return fail(PJ_ERR_HIP, "hipGetSymbolAddress returned an unusable scratch address");
+    g_scratch_ptr[dev] = p;
+  }
+  *ptr = g_scratch_ptr[dev];
+  return PJ_OK;
+}
+
+}  // namespace pj
+
+extern "C" {
+
+pj_status pj_partitioned_join_noalloc(const pj_table* build, const pj_table* probe, const pj_plan* plan,
+                                      pj_pair* out, int64_t out_capacity, int64_t* d_out_count, hipStream_t stream) {
+  size_t need = 0;
+  pj_status st = pj_workspace_size(build, probe, plan, &need);
+  if (st != PJ_OK) return st;
+  if (need > kNoAllocScratchBytes)
+    return fail(PJ_ERR_WORKSPACE,
+                "this join needs %zu bytes of workspace; the allocation-free entry point has %zu (use "
+                "pj_partitioned_join_async with a caller workspace)",
+                need, kNoAllocScratchBytes);
+  void* ws = nullptr;
+  st = scratch_address(&ws);
+  if (st != PJ_OK) return st;
+  return run_join(build, probe, plan, out, out_capacity, d_out_count, ws, kNoAllocScratchBytes, stream);
 }
 
 pj_status pj_project_reference(const pj_table* build, const pj_pair* pairs, int64_t n, void* out_xy,

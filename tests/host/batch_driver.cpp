@@ -11,6 +11,7 @@
 //   eager_short                 the same, but the workspace registered with the
 //                               stand-in runtime is 4 KB shorter than the size
 //                               passed (a planted mistake the checks must catch)
+//   noalloc                     pj_partitioned_join_noalloc of every join
 //   create ID CHAINS null|plans then, for "plans", N lines
 //                               "passes bits1 bits2 factor unique slice source"
 //   launch ID REPS
@@ -106,6 +107,17 @@ int main() {
         if (st != PJ_OK && st != PJ_ERR_CAPACITY) expect_ok(st, "pj_partitioned_join");
         dev_free(out);
         dev_free(ws);
+      }
+    } else if (!strcmp(cmd, "noalloc")) {
+      for (int j = 0; j < n; ++j) {
+        const int64_t cap = tp[j].n > 0 ? tp[j].n : 1;
+        void* out = dev_alloc(size_t(cap) * 8, "noalloc output");
+        void* cnt = dev_alloc(8, "noalloc count");
+        const pj_status st = pj_partitioned_join_noalloc(&tb[j], &tp[j], nullptr, static_cast<pj_pair*>(out), cap,
+                                                         static_cast<int64_t*>(cnt), nullptr);
+        if (st != PJ_ERR_WORKSPACE) expect_ok(st, "pj_partitioned_join_noalloc");
+        dev_free(out);
+        dev_free(cnt);
       }
     } else if (!strcmp(cmd, "create")) {
       int id, chains;

@@ -121,3 +121,11 @@ def test_checks_catch_a_short_workspace(driver):
     r = subprocess.run([driver], input=joins_block([(100000, 300000)]) + "eager_short\n", capture_output=True,
                        text=True, env=env, timeout=120)
     assert r.returncode == 1 and "overruns eager workspace (short)" in r.stderr, r.stderr[-2000:]
+
+
+def test_noalloc_entry_point_stays_in_its_scratch(driver):
+    # the static scratch is registered with the stand-in runtime at its declared size;
+    # joins from 1 row to the largest that fits (and one that does not: rejected)
+    sizes = [(1, 1), (1000, 7), (1 << 20, 1 << 22), (4 << 20, 8 << 20), (16 << 20, 64 << 20), (0, 5)]
+    out = run(driver, joins_block(sizes, 4) + "noalloc\n" + joins_block(sizes, 8) + "noalloc\n")
+    assert "launch errors 0" in out and "call failures 0" in out
