@@ -105,20 +105,41 @@ def batched_joins(pj, torch, count=512):
     bj.close()
     t0 = time.perf_counter()
     best, cfg, timings = batch.autotune(builds, probes, plain, chain_options=(8, 16, 32), bit_offsets=(-1, 0, 1),
-                                        reps=3)
+                                        tiles=((None, None), (1.0, 8192)), reps=3)
     tune_s = time.perf_counter() - t0
     ms_best = best.time(5)
     counts_best = best.counts()
     best.close()
     assert np.array_equal(counts_default, counts_best)
+    # the same joins one after another on one stream (no graph, plain plans,
+    # one shared workspace): what batching is measured against
+    ws = torch.empty(max(pj.workspace_size(b.numel(), p.numel(), 8, pl) for b, p, pl in zip(builds, probes, plain)),
+                     dtype=torch.uint8, device="cuda")
+    outs = [torch.empty((max(1, p.numel()), 2), dtype=torch.int32, device="cuda") for p in probes]
+    def sequential():
+        return [pj.partitioned_join_async(b, p, o, plan=pl, workspace=ws) for b, p, o, pl in zip(builds, probes, outs,
+                                                                                                 plain)]
+    sequential()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(3):
+        seq_counts = sequential()
+    torch.cuda.synchronize()
+    ms_seq = (time.perf_counter() - t0) * 1000.0 / 3
+    assert np.array_equal(np.array([int(c.item()) for c in seq_counts]), counts_best)
+    del ws, outs
     return {"workload": f"{count} joins, |R| log-uniform [10K,4M], |S| 1-8x|R|, uniform 64-bit int64 build keys, "
                         "Zipf(theta~U[0,1.2]) probes, int32 row-id payloads, one HIP graph",
             "tuples": tuples, "pairs": int(counts_best.sum()),
             "default_ms": ms_default, "default_tuples_per_s": tuples / (ms_default / 1000.0),
             "tuned_ms": ms_best, "tuned_tuples_per_s": tuples / (ms_best / 1000.0),
-            "tuned_config": {"chains": cfg[0], "fanout_offset_bits": cfg[1], "plans": "plain"},
+            "sequential_ms": ms_seq, "sequential_tuples_per_s": tuples / (ms_seq / 1000.0),
+            "tuned_config": {"chains": cfg[0], "fanout_offset_bits": cfg[1],
+                             "join_tile": "plan's own" if len(cfg) == 2 else {"chunk_factor": cfg[2][0],
+                                                                               "probe_slice": cfg[2][1]},
+                             "plans": "plain"},
             "autotune_seconds": tune_s,
-            "timings_ms": {f"chains={c},fanout{d:+d}": v for (c, d), v in timings.items()}}
+            "timings_ms": {",".join(str(x) for x in k): v for k, v in timings.items()}}
 
 
 def batched_joins_child(timeout_s=600):

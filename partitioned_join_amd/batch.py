@@ -1,6 +1,6 @@
 """Batched joins: many independent joins of uneven sizes replayed as one HIP
 graph (pj_batch_* in libpj.so), with an autotuner for the number of parallel
-chains and the radix fan-out.
+chains, the radix fan-out and the join's work-item tile.
 
 The joins are spread longest-first over parallel chains that are captured
 once into one graph (pj_batch_create). Each chain reuses one workspace region
@@ -8,7 +8,8 @@ for its joins. One pj_batch_launch then replays everything. The autotuner
 times whole-batch replays over:
 - chain counts;
 - a fan-out offset (-1, 0 or +1 total radix bits) applied to every join's
-  plan.
+  plan;
+- the join's work-item tile: the build chunk factor and the probe slice.
 
 It keeps the fastest combination. Plans come from the plain planner, or from
 the learned planner per join, with its plain-plan fallback.
@@ -164,26 +165,43 @@ def batch_plans(builds, probes, planner: str = "plain") -> List[PjPlan]:
     return plans
 
 
+def tile_plan(plan: PjPlan, factor: Optional[float], probe_slice: Optional[int]) -> PjPlan:
+    """The plan with the join's work-item tile changed: the build chunk factor
+    f (chunk = ceil(f |R| / P), within the LDS table) and the probe slice
+    (0 = automatic); None keeps the plan's own."""
+    p = PjPlan()
+    ctypes.memmove(ctypes.byref(p), ctypes.byref(plan), ctypes.sizeof(PjPlan))
+    if factor is not None:
+        p.max_partition_size_factor = float(factor)
+    if probe_slice is not None:
+        p.probe_slice = int(probe_slice)
+    return p
+
+
 def autotune(builds, probes, base_plans: Sequence[PjPlan], chain_options=(4, 8, 16, 32), bit_offsets=(-1, 0, 1),
-             reps: int = 3, log=None):
-    """Times the batch under every (chains, fan-out offset) and returns
-    (best BatchJoin, best config, all timings in ms)."""
+             tiles=((None, None),), reps: int = 3, log=None):
+    """Times the batch under every (chains, fan-out offset, join tile) and
+    returns (best BatchJoin, best config, all timings in ms).  A tile is
+    (build chunk factor, probe slice), None keeping each plan's own."""
     timings = {}
     best = None
     for delta in bit_offsets:
-        plans = [shift_plan(p, delta, b.numel(), b.element_size()) for p, b in zip(base_plans, builds)]
-        for chains in chain_options:
-            bj = BatchJoin(builds, probes, plans, chains=chains)
-            ms = bj.time(reps)
-            timings[(chains, delta)] = ms
-            if log:
-                log(f"chains={chains} fanout{delta:+d}: {ms:.2f} ms")
-            if best is None or ms < best[0]:
-                if best is not None:
-                    best[1].close()
-                best = (ms, bj, (chains, delta))
-            else:
-                bj.close()
+        shifted = [shift_plan(p, delta, b.numel(), b.element_size()) for p, b in zip(base_plans, builds)]
+        for tile in tiles:
+            plans = [tile_plan(p, *tile) for p in shifted]
+            for chains in chain_options:
+                bj = BatchJoin(builds, probes, plans, chains=chains)
+                ms = bj.time(reps)
+                key = (chains, delta) if tile == (None, None) else (chains, delta, tile)
+                timings[key] = ms
+                if log:
+                    log(f"chains={chains} fanout{delta:+d} tile={tile}: {ms:.2f} ms")
+                if best is None or ms < best[0]:
+                    if best is not None:
+                        best[1].close()
+                    best = (ms, bj, key)
+                else:
+                    bj.close()
     return best[1], best[2], timings
 
 

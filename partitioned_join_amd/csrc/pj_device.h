@@ -174,18 +174,6 @@ __device__ __forceinline__ uint32_t mbcnt(uint64_t mask) {
   return __builtin_amdgcn_mbcnt_hi(uint32_t(mask >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(mask), 0u));
 }
 
-// Lanes of the wave whose `bits`-bit digit equals this lane's, among lanes
-// with valid set (one ballot per digit bit).  Must be called by all lanes.
-__device__ __forceinline__ uint64_t match_digit(uint32_t d, int bits, bool valid) {
-  uint64_t peers = __ballot(valid);
-  for (int b = 0; b < bits; ++b) {
-    const bool set = (d >> b) & 1u;
-    const uint64_t m = __ballot(set);
-    peers &= set ? m : ~m;
-  }
-  return valid ? peers : 0ull;
-}
-
 // Block-wide exclusive scan of one uint32 per thread.  s_tmp needs BLOCK/64 + 1
 // entries.  Returns the exclusive prefix; *total receives the block sum.
 // Contains two LDS barriers.

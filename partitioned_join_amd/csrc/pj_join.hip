@@ -819,12 +819,18 @@ hipError_t launch_join(const Layout& L, int key_bytes, int base, int64_t slice, 
   return launch_join_kernel(A, key_bytes, base, unique, digit_sentinel, s);
 }
 
+// Workgroups per stray run in the output defrag, each copying one slice of
+// it: the copy is latency-bound (a search, then a walk over the holes), so
+// more, shorter slices finish sooner (compact 0.065 -> 0.037 ms at 64M x 256M
+// going from 4 to 16).
+constexpr int kDefragSplit = 16;
+
 hipError_t launch_defrag(const Layout& L, pj_pair* out, int64_t capacity, int64_t* d_count, hipStream_t s) {
   hipLaunchKernelGGL(k_defrag_plan, dim3(1), dim3(1024), 0, s, L.chunk_ctr, L.wg_last, kJoinGrid, capacity, d_count,
                      L.dstate, L.hole_lo, L.hole_pre, L.stray_lo, L.stray_pre);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_defrag_move, dim3(kJoinGrid + 1, 4), dim3(256), 0, s, L.dstate, L.hole_lo, L.hole_pre, L.stray_lo,
+  hipLaunchKernelGGL(k_defrag_move, dim3(kJoinGrid + 1, kDefragSplit), dim3(256), 0, s, L.dstate, L.hole_lo, L.hole_pre, L.stray_lo,
                      L.stray_pre, reinterpret_cast<uint2*>(out), capacity, reinterpret_cast<const uint2*>(L.spill));
   return hipGetLastError();
 }

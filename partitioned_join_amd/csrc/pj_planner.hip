@@ -122,8 +122,18 @@ __global__ __launch_bounds__(1024) void k_sketch(const void* keys, int64_t n, in
 
 // ---------------------------------------------------------------------------
 // Exact candidate counts: the valid candidates (sample count > 0) sit in a
-// 128-slot LDS open-addressing table; every probe key is looked up once.
+// 128-slot LDS open-addressing table; every probe key is looked up once.  The
+// table's slot is the top bits of a multiplicative hash (one multiply per
+// key; the counts do not depend on it).
 // ---------------------------------------------------------------------------
+template <int SLOTS>
+__device__ __forceinline__ uint32_t cand_slot(uint32_t k) {
+  return (k * 0x9E3779B1u) >> (32 - __builtin_ctz(uint32_t(SLOTS)));
+}
+template <int SLOTS>
+__device__ __forceinline__ uint32_t cand_slot(uint64_t k) {
+  return ((uint32_t(k) ^ (uint32_t(k >> 32) * 0x85EBCA77u)) * 0x9E3779B1u) >> (32 - __builtin_ctz(uint32_t(SLOTS)));
+}
 template <int KB>
 __global__ __launch_bounds__(256) void k_count_cand(const void* keys, int64_t n, const uint64_t* cand_keys,
                                                     const uint32_t* cand_sample_counts,
@@ -141,7 +151,7 @@ __global__ __launch_bounds__(256) void k_count_cand(const void* keys, int64_t n,
     for (int c = 0; c < kPlanCand; ++c) {
       if (cand_sample_counts[c] == 0) continue;
       const key_t key = key_t(cand_keys[c]);
-      uint32_t s = murmur3(key) & uint32_t(SLOTS - 1);
+      uint32_t s = cand_slot<SLOTS>(key);
       while (s_idx[s] >= 0) s = (s + 1) & uint32_t(SLOTS - 1);
       s_key[s] = uint64_t(key);
       s_idx[s] = c;
@@ -153,7 +163,7 @@ __global__ __launch_bounds__(256) void k_count_cand(const void* keys, int64_t n,
     const key_t key = k[i];
     uint64_t v = uint64_t(key);
     if (KB == 4) v &= 0xffffffffull;
-    uint32_t s = murmur3(key) & uint32_t(SLOTS - 1);
+    uint32_t s = cand_slot<SLOTS>(key);
     while (s_idx[s] >= 0) {  // at most 64 of the 128 slots are used
       if (s_key[s] == v) {
         atomicAdd(&s_cnt[s_idx[s]], 1u);
@@ -190,7 +200,7 @@ __global__ __launch_bounds__(256) void k_count_cand_vec(const void* keys, int64_
     for (int c = 0; c < kPlanCand; ++c) {
       if (cand_sample_counts[c] == 0) continue;
       const key_t key = key_t(cand_keys[c]);
-      uint32_t s = murmur3(key) & uint32_t(SLOTS - 1);
+      uint32_t s = cand_slot<SLOTS>(key);
       while (s_idx[s] >= 0) s = (s + 1) & uint32_t(SLOTS - 1);
       s_key[s] = uint64_t(key);
       s_idx[s] = c;
@@ -200,7 +210,7 @@ __global__ __launch_bounds__(256) void k_count_cand_vec(const void* keys, int64_
   auto lookup = [&](key_t key) {
     uint64_t v = uint64_t(key);
     if (KB == 4) v &= 0xffffffffull;
-    uint32_t s = murmur3(key) & uint32_t(SLOTS - 1);
+    uint32_t s = cand_slot<SLOTS>(key);
     while (s_idx[s] >= 0) {  // at most 64 of the slots are used
       if (s_key[s] == v) {
         atomicAdd(&s_cnt[s_idx[s]], 1u);

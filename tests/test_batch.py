@@ -138,8 +138,8 @@ def test_batch_autotune_and_learned_plans(pj, oracle, torch):
 def test_batch_shaped_like_the_benchmark_against_the_oracle(pj, oracle, torch):
     """64 joins of the batched benchmark's workload (|R| log-uniform from 10K,
     |S| 1-8x |R|, uniform 64-bit build keys, Zipf probes), sizes capped for the
-    CPU oracle, plain plans and the autotuner's chain / fan-out search: every
-    join's pairs of the tuned batch equal the oracle's join."""
+    CPU oracle, plain plans and the autotuner's chain / fan-out / join-tile
+    search: every join's pairs of the tuned batch equal the oracle's join."""
     from partitioned_join_amd import batch
     builds, probes, specs = batch.make_batch_workload(64, seed=0xC5)
     keep = [j for j, sp in enumerate(specs) if sp["n_build"] <= 400000 and sp["n_probe"] <= 2000000]
@@ -147,7 +147,9 @@ def test_batch_shaped_like_the_benchmark_against_the_oracle(pj, oracle, torch):
     probes = [probes[j] for j in keep]
     assert len(builds) >= 24
     plans = batch.batch_plans(builds, probes, "plain")
-    bj, cfg, timings = batch.autotune(builds, probes, plans, chain_options=(4, 8), bit_offsets=(0, 1), reps=1)
+    bj, cfg, timings = batch.autotune(builds, probes, plans, chain_options=(4, 8), bit_offsets=(0, 1),
+                                      tiles=((None, None), (1.0, 8192)), reps=1)
+    assert len(timings) == 8 and cfg in timings
     bj.launch()
     torch.cuda.synchronize()
     counts = bj.counts()
