@@ -334,7 +334,9 @@ pj_status pj_planner_train_step(float* d_weights, float* d_adam_m, float* d_adam
 pj_status pj_plan_decode(const float* logits, const uint32_t* cand_sample_counts, const uint64_t* cand_exact_counts,
                          const pj_table* build, const pj_table* probe, pj_plan* plan, int32_t* fallback);
 size_t pj_plan_learned_workspace_size(void);
-/* Sketch, forward pass, exact candidate counts, then pj_plan_decode.
+/* Sketch, forward pass, exact candidate counts, then pj_plan_decode.  When the
+ * predicted plan is the plain plan itself the heavy-set check has nothing to
+ * choose between, so the counts are skipped (*fallback = 0, plan = that plan).
  * Synchronises `stream`. */
 pj_status pj_plan_learned(const pj_table* build, const pj_table* probe, const float* d_weights, pj_plan* plan,
                           int32_t* fallback, void* workspace, size_t workspace_bytes, hipStream_t stream);

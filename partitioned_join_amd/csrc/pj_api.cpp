@@ -888,17 +888,54 @@ size_t pj_plan_learned_workspace_size(void) {
   return c.off;
 }
 
-pj_status pj_plan_decode(const float* logits, const uint32_t* cand_sample_counts, const uint64_t* cand_exact_counts,
-                         const pj_table* build, const pj_table* probe, pj_plan* plan, int32_t* fallback) {
-  pj_status st = check_pair(build, probe, false);
-  if (st != PJ_OK) return st;
-  if (!logits || !cand_sample_counts || !cand_exact_counts || !plan) return fail(PJ_ERR_INVALID_ARG, "NULL argument");
+}  // extern "C"
+
+namespace pj {
+
+// The plan the logits predict (bits and pass count; the rest as the plain
+// plan), and whether it may be taken at all: its build chunks must fit the
+// LDS table.
+static bool learned_plan_of(const float* logits, const pj_table* build, pj_plan* learned) {
   auto argmax = [&](int off, int n) {
     int best = 0;
     for (int c = 1; c < n; ++c)
       if (logits[off + c] > logits[off + best]) best = c;
     return best;
   };
+  plain_plan(build->n, build->key_bytes, learned);
+  const int passes = 1 + argmax(2 * kPlanBitsCls, kPlanPassCls);
+  const int b1 = 6 + argmax(0, kPlanBitsCls), b2 = 6 + argmax(kPlanBitsCls, kPlanBitsCls);
+  if (passes == 1) {
+    learned->passes = 1;
+    learned->bits1 = b1;
+    learned->bits2 = 0;
+  } else {  // three passes are not implemented: two passes with the predicted bits
+    learned->passes = 2;
+    learned->bits1 = b1;
+    learned->bits2 = b1 + b2 > kMaxCombinedBits ? kMaxCombinedBits - b1 : b2;
+  }
+  learned->source = 1;
+  const int cap = join_table_capacity(build->key_bytes) / 2 - 1;
+  const double per = learned->max_partition_size_factor * double(build->n) /
+                     double(int64_t(1) << (learned->bits1 + learned->bits2));
+  return per <= double(cap) && check_plan(learned) == PJ_OK;
+}
+
+static bool same_plan(const pj_plan& a, const pj_plan& b) {
+  return a.passes == b.passes && a.bits1 == b.bits1 && a.bits2 == b.bits2 &&
+         a.max_partition_size_factor == b.max_partition_size_factor && a.probe_slice == b.probe_slice &&
+         a.unique_build_keys == b.unique_build_keys;
+}
+
+}  // namespace pj
+
+extern "C" {
+
+pj_status pj_plan_decode(const float* logits, const uint32_t* cand_sample_counts, const uint64_t* cand_exact_counts,
+                         const pj_table* build, const pj_table* probe, pj_plan* plan, int32_t* fallback) {
+  pj_status st = check_pair(build, probe, false);
+  if (st != PJ_OK) return st;
+  if (!logits || !cand_sample_counts || !cand_exact_counts || !plan) return fail(PJ_ERR_INVALID_ARG, "NULL argument");
   // Heavy key: at least T = |S| / 1024 probe rows.  The predicted set
   // disagrees with the exact counts when a key predicted heavy has fewer than
   // T / 2 rows or a key not predicted heavy has 2T or more; keys within a
@@ -915,25 +952,8 @@ pj_status pj_plan_decode(const float* logits, const uint32_t* cand_sample_counts
     if (!pred && c * 512ull >= ns && ns > 0) agree = false;  // missed, at least 2T
   }
   pj_plan learned;
-  plain_plan(build->n, build->key_bytes, &learned);
-  const int passes = 1 + argmax(2 * kPlanBitsCls, kPlanPassCls);
-  const int b1 = 6 + argmax(0, kPlanBitsCls), b2 = 6 + argmax(kPlanBitsCls, kPlanBitsCls);
-  if (passes == 1) {
-    learned.passes = 1;
-    learned.bits1 = b1;
-    learned.bits2 = 0;
-  } else {  // three passes are not implemented: two passes with the predicted bits
-    learned.passes = 2;
-    learned.bits1 = b1;
-    learned.bits2 = b1 + b2 > kMaxCombinedBits ? kMaxCombinedBits - b1 : b2;
-  }
-  learned.source = 1;
-  // a plan whose build chunks would not fit the LDS table is never taken
-  const int cap = join_table_capacity(build->key_bytes) / 2 - 1;
-  const double per = learned.max_partition_size_factor * double(build->n) /
-                     double(int64_t(1) << (learned.bits1 + learned.bits2));
-  const bool fits = per <= double(cap);
-  if (agree && fits && check_plan(&learned) == PJ_OK) {
+  const bool fits = learned_plan_of(logits, build, &learned);
+  if (agree && fits) {
     *plan = learned;
     if (fallback) *fallback = 0;
   } else {
@@ -959,15 +979,28 @@ pj_status pj_plan_learned(const pj_table* build, const pj_table* probe, const fl
   float* scratch = static_cast<float*>(c.take(planner_scratch_floats(1) * sizeof(float)));
   PJ_HIP(planner_sketch(probe->keys, probe->n, probe->key_bytes, build->n, feat, cand, cand_n, stream));
   PJ_HIP(planner_forward(d_weights, feat, 1, logits, scratch, stream));
+  float h_logits[kPlanOut];
+  PJ_HIP(hipMemcpyAsync(h_logits, logits, sizeof(h_logits), hipMemcpyDeviceToHost, stream));
+  PJ_HIP(hipStreamSynchronize(stream));
+  // The heavy-set check chooses between the learned and the plain plan; when
+  // the model predicts the plain plan itself there is nothing to choose, and
+  // the exact candidate counts (a read of the whole probe column) are skipped.
+  pj_plan learned, plain;
+  const bool fits = learned_plan_of(h_logits, build, &learned);
+  plain_plan(build->n, build->key_bytes, &plain);
+  if (fits && same_plan(learned, plain)) {
+    *plan = learned;
+    if (fallback) *fallback = 0;
+    return PJ_OK;
+  }
   PJ_HIP(planner_count_candidates(probe->keys, probe->n, probe->key_bytes, cand, cand_n, exact, stream));
-  // cand_n, exact and logits lie in that order in the workspace: one copy
+  // cand_n and exact lie in that order in the workspace: one copy
   const char* lo = reinterpret_cast<const char*>(cand_n);
-  const size_t span = size_t(reinterpret_cast<const char*>(logits + kPlanOut) - lo);
-  alignas(16) char h_span[((kPlanCand * 4 + 255) & ~255) + ((kPlanCand * 8 + 255) & ~255) + kPlanOut * sizeof(float)];
+  const size_t span = size_t(reinterpret_cast<const char*>(exact + kPlanCand) - lo);
+  alignas(16) char h_span[((kPlanCand * 4 + 255) & ~255) + kPlanCand * 8];
   if (span != sizeof(h_span)) return fail(PJ_ERR_UNSUPPORTED, "planner workspace layout changed");
   PJ_HIP(hipMemcpyAsync(h_span, lo, span, hipMemcpyDeviceToHost, stream));
   PJ_HIP(hipStreamSynchronize(stream));
-  const float* h_logits = reinterpret_cast<const float*>(h_span + (reinterpret_cast<const char*>(logits) - lo));
   const uint32_t* h_cand_n = reinterpret_cast<const uint32_t*>(h_span);
   const uint64_t* h_exact = reinterpret_cast<const uint64_t*>(h_span + (reinterpret_cast<const char*>(exact) - lo));
   return pj_plan_decode(h_logits, h_cand_n, h_exact, build, probe, plan, fallback);

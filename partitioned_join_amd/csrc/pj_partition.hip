@@ -705,7 +705,7 @@ __host__ __device__ constexpr size_t scatter_lds_bytes(int F) {
 //   4-byte keys  F <= 64: 16K + carry; F <= 256: 12K (pass 1) or 14K (pass 2)
 //                + carry; 512: 8K + carry; 1024: 16K; more: 8K;
 //   8-byte keys  (16-byte tuples) F <= 64: 8K + carry (pass 2), F <= 128: 8K +
-//                carry (pass 1); F <= 256: 4K (pass 1) or 7K (pass 2) + carry;
+//                carry (pass 1); F <= 256: 6K (pass 1) or 7K (pass 2) + carry;
 //                512: 4K + carry; 1024: 8K; more: 4K.
 // Pass 2 of a deep plan with 4096 x 4096 partitions: the item prefix over
 // 4096 pass-1 partitions leaves room for 7K (4-byte keys) or 3K tuples.
@@ -722,7 +722,7 @@ inline ScatterCfg scatter_cfg(int key_bytes, int F, bool pass2, int F1 = 0) {
     return {8192, 0};
   }
   if (F <= (pass2 ? 64 : 128)) return {8192, pass2 ? 64 : 128};
-  if (F <= 256) return {pass2 ? 7168 : 4096, pass2 ? 256 : 512};
+  if (F <= 256) return {pass2 ? 7168 : 6144, 256};
   if (F == 512) return {4096, 512};
   if (F == 1024) return {8192, 0};
   return {4096, 0};
@@ -741,7 +741,8 @@ __device__ __forceinline__ void pass1_body(const PartTable& T, const PartParams&
   using key_t = typename KTr::key_t;
   using tup_t = typename ScatterTup<KB>::type;  // the tuple as a vector
   constexpr int E = TILE / BLOCK;
-  static_assert(E % 4 == 0, "tile must give each thread a multiple of 4 items");
+  constexpr int G = KB == 4 ? 4 : 2;  // keys per 16-byte load
+  static_assert(E % G == 0, "tile must give each thread whole 16-byte groups of keys");
   const uint32_t S = T.nseg / kNumXcd;
   const uint32_t seg = (b % kNumXcd) * S + b / kNumXcd;
   const uint32_t lo = seg_bound(T.n, T.nseg, seg), hi = seg_bound(T.n, T.nseg, seg + 1);
@@ -774,57 +775,60 @@ __device__ __forceinline__ void pass1_body(const PartTable& T, const PartParams&
   };
   const key_t* keys = static_cast<const key_t*>(T.keys);
 
-  // Item e4 * 4 + c of a thread is row tile_lo + (e4 * BLOCK + tid) * 4 + c.
+  // Item g * G + c of a thread is row tile_lo + (g * BLOCK + tid) * G + c:
+  // one 16-byte load of keys per group (payloads: G * 4 bytes).
   auto load_full = [&](uint32_t tile_lo, key_t (&key)[E], int32_t (&pay)[E]) {
 #pragma unroll
-    for (int e4 = 0; e4 < E / 4; ++e4) {
-      const uint32_t i = tile_lo + (uint32_t(e4) * BLOCK + tid) * 4;
+    for (int g = 0; g < E / G; ++g) {
+      const uint32_t i = tile_lo + (uint32_t(g) * BLOCK + tid) * G;
       if (VEC) {
-        if (KB == 4) {
-          const u32x4 v = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(keys + i));
-          key[e4 * 4 + 0] = key_t(v.x); key[e4 * 4 + 1] = key_t(v.y);
-          key[e4 * 4 + 2] = key_t(v.z); key[e4 * 4 + 3] = key_t(v.w);
+        const u32x4 v = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(keys + i));
+        if constexpr (KB == 4) {
+          key[g * 4 + 0] = key_t(v.x); key[g * 4 + 1] = key_t(v.y);
+          key[g * 4 + 2] = key_t(v.z); key[g * 4 + 3] = key_t(v.w);
         } else {
-          const u32x4 a = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(keys + i));
-          const u32x4 c = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(keys + i + 2));
-          key[e4 * 4 + 0] = key_t(uint64_t(a.x) | (uint64_t(a.y) << 32));
-          key[e4 * 4 + 1] = key_t(uint64_t(a.z) | (uint64_t(a.w) << 32));
-          key[e4 * 4 + 2] = key_t(uint64_t(c.x) | (uint64_t(c.y) << 32));
-          key[e4 * 4 + 3] = key_t(uint64_t(c.z) | (uint64_t(c.w) << 32));
+          key[g * 2 + 0] = key_t(uint64_t(v.x) | (uint64_t(v.y) << 32));
+          key[g * 2 + 1] = key_t(uint64_t(v.z) | (uint64_t(v.w) << 32));
         }
         if (PAY) {
-          const u32x4 v = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(T.pay + i));
-          pay[e4 * 4 + 0] = int32_t(v.x); pay[e4 * 4 + 1] = int32_t(v.y);
-          pay[e4 * 4 + 2] = int32_t(v.z); pay[e4 * 4 + 3] = int32_t(v.w);
+          if constexpr (G == 4) {
+            const u32x4 q = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(T.pay + i));
+            pay[g * 4 + 0] = int32_t(q.x); pay[g * 4 + 1] = int32_t(q.y);
+            pay[g * 4 + 2] = int32_t(q.z); pay[g * 4 + 3] = int32_t(q.w);
+          } else {
+            typedef uint32_t u32x2v __attribute__((ext_vector_type(2)));
+            const u32x2v q = __builtin_nontemporal_load(reinterpret_cast<const u32x2v*>(T.pay + i));
+            pay[g * 2 + 0] = int32_t(q.x); pay[g * 2 + 1] = int32_t(q.y);
+          }
         }
       } else {
 #pragma unroll
-        for (int c = 0; c < 4; ++c) {
-          key[e4 * 4 + c] = __builtin_nontemporal_load(keys + i + c);
-          if (PAY) pay[e4 * 4 + c] = __builtin_nontemporal_load(T.pay + i + c);
+        for (int c = 0; c < G; ++c) {
+          key[g * G + c] = __builtin_nontemporal_load(keys + i + c);
+          if (PAY) pay[g * G + c] = __builtin_nontemporal_load(T.pay + i + c);
         }
       }
     }
   };  // without a payload column pay is left alone: the row id is formed when staging
   auto load_tail = [&](uint32_t tile_lo, key_t (&key)[E], int32_t (&pay)[E]) {
 #pragma unroll
-    for (int e4 = 0; e4 < E / 4; ++e4) {
-      const uint32_t i = tile_lo + (uint32_t(e4) * BLOCK + tid) * 4;
+    for (int g = 0; g < E / G; ++g) {
+      const uint32_t i = tile_lo + (uint32_t(g) * BLOCK + tid) * G;
 #pragma unroll
-      for (int c = 0; c < 4; ++c) {
+      for (int c = 0; c < G; ++c) {
         const bool valid = i + c < hi;
-        key[e4 * 4 + c] = valid ? keys[i + c] : key_t(0);
-        pay[e4 * 4 + c] = PAY ? (valid ? T.pay[i + c] : 0) : int32_t(i + c);
+        key[g * G + c] = valid ? keys[i + c] : key_t(0);
+        pay[g * G + c] = PAY ? (valid ? T.pay[i + c] : 0) : int32_t(i + c);
       }
     }
   };
   auto digits = [&](uint32_t tile_lo, const key_t (&key)[E], uint32_t (&dg)[E]) {
 #pragma unroll
-    for (int e4 = 0; e4 < E / 4; ++e4) {
-      const uint32_t i = tile_lo + (uint32_t(e4) * BLOCK + tid) * 4;
+    for (int g = 0; g < E / G; ++g) {
+      const uint32_t i = tile_lo + (uint32_t(g) * BLOCK + tid) * G;
 #pragma unroll
-      for (int c = 0; c < 4; ++c)
-        dg[e4 * 4 + c] = i + c < hi ? digit_of(KTr::low32(key[e4 * 4 + c]), shift, mask) : 0xFFFFFFFFu;
+      for (int c = 0; c < G; ++c)
+        dg[g * G + c] = i + c < hi ? digit_of(KTr::low32(key[g * G + c]), shift, mask) : 0xFFFFFFFFu;
     }
   };
 
@@ -857,7 +861,7 @@ __device__ __forceinline__ void pass1_body(const PartTable& T, const PartParams&
     const uint32_t tile_lo = lo + t * uint32_t(TILE);
 #pragma unroll
     for (int e = 0; e < E; ++e)
-      tup[e] = ScatterTup<KB>::make(ck[e], PAY ? cp[e] : int32_t(tile_lo + (uint32_t(e / 4) * BLOCK + tid) * 4 + (e % 4)));
+      tup[e] = ScatterTup<KB>::make(ck[e], PAY ? cp[e] : int32_t(tile_lo + (uint32_t(e / G) * BLOCK + tid) * G + (e % G)));
     scan_stage_tile<KB, BLOCK, E, CF>(tup, pk, F, s_stage, s_cnt, s_cnt_next, s_base, s_cur, s_delta, cy,
                                       no_tail && t + 1 == nfull, cr);
     swap_counts();
@@ -930,6 +934,7 @@ static hipError_t pass1_launch(const Layout& L, hipStream_t s) {
     PJ_P1(8192, 0);
   } else {
     PJ_P1(8192, 128);
+    PJ_P1(6144, 256);
     PJ_P1(4096, 512);
     PJ_P1(8192, 0);
     PJ_P1(4096, 0);

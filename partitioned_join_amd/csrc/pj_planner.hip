@@ -442,7 +442,67 @@ static hipError_t forward(const float* w, const float* x, int batch, float* h1, 
   return e;
 }
 
+// One sample (planning a join): matrix-vector products split over K, so
+// every layer reads its weights with many workgroups.  Workgroup w of a layer
+// takes input rows [w * KC, (w + 1) * KC) and writes partial sums part[w][n];
+// the next layer first adds its input's partials, the bias and the ReLU.
+// Fixed split and summation order: the logits are deterministic.
+constexpr int kGemvBlock = 256;
+constexpr int kGemvSplit1 = 16, kGemvSplit2 = 8, kGemvSplit3 = 8;
+
+__global__ __launch_bounds__(kGemvBlock) void k_gemv(const float* x, const float* x_parts, int x_nparts,
+                                                     const float* x_bias, const float* W, int K, int N, int KC,
+                                                     float* out_parts) {
+  __shared__ float s_x[128];
+  const int k0 = blockIdx.x * KC, k1 = min(K, k0 + KC);
+  for (int k = k0 + threadIdx.x; k < k1; k += kGemvBlock) {
+    float v;
+    if (x_parts) {  // hidden layer input: sum of partials + bias, ReLU
+      v = x_bias[k];
+      for (int p = 0; p < x_nparts; ++p) v += x_parts[size_t(p) * K + k];
+      v = v > 0.f ? v : 0.f;
+    } else {
+      v = x[k];
+    }
+    s_x[k - k0] = v;
+  }
+  __syncthreads();
+  for (int n = threadIdx.x; n < N; n += kGemvBlock) {
+    float acc = 0.f;
+    for (int k = k0; k < k1; ++k) acc = fmaf(s_x[k - k0], W[size_t(k) * N + n], acc);
+    out_parts[size_t(blockIdx.x) * N + n] = acc;
+  }
+}
+
+__global__ __launch_bounds__(kGemvBlock) void k_gemv_final(const float* parts, int nparts, const float* bias, int N,
+                                                           float* out) {
+  for (int n = threadIdx.x; n < N; n += kGemvBlock) {
+    float v = bias[n];
+    for (int p = 0; p < nparts; ++p) v += parts[size_t(p) * N + n];
+    out[n] = v;
+  }
+}
+
+static hipError_t forward_one(const float* w, const float* x, float* out, float* scratch, hipStream_t s) {
+  const int F = kPlanFeat, H = kPlanHidden, O = kPlanOut;
+  float* p1 = scratch;                                  // [kGemvSplit1][H]
+  float* p2 = p1 + size_t(kGemvSplit1) * H;             // [kGemvSplit2][H]
+  float* p3 = p2 + size_t(kGemvSplit2) * H;             // [kGemvSplit3][O]
+  const int kc1 = (F + kGemvSplit1 - 1) / kGemvSplit1, kc2 = H / kGemvSplit2, kc3 = H / kGemvSplit3;
+  hipLaunchKernelGGL(k_gemv, dim3(kGemvSplit1), dim3(kGemvBlock), 0, s, x, nullptr, 0, nullptr, w + kPlanW1, F, H,
+                     kc1, p1);
+  hipLaunchKernelGGL(k_gemv, dim3(kGemvSplit2), dim3(kGemvBlock), 0, s, nullptr, p1, kGemvSplit1, w + kPlanB1,
+                     w + kPlanW2, H, H, kc2, p2);
+  hipLaunchKernelGGL(k_gemv, dim3(kGemvSplit3), dim3(kGemvBlock), 0, s, nullptr, p2, kGemvSplit2, w + kPlanB2,
+                     w + kPlanW3, H, O, kc3, p3);
+  hipLaunchKernelGGL(k_gemv_final, dim3(1), dim3(kGemvBlock), 0, s, p3, kGemvSplit3, w + kPlanB3, O, out);
+  return hipGetLastError();
+}
+static_assert((kPlanFeat + kGemvSplit1 - 1) / kGemvSplit1 <= 128 && kPlanHidden / kGemvSplit2 <= 128,
+              "a workgroup's input rows fit its LDS slice");
+
 hipError_t planner_forward(const float* w, const float* x, int batch, float* out, float* scratch, hipStream_t s) {
+  if (batch == 1) return forward_one(w, x, out, scratch, s);
   float* h1 = scratch;
   float* h2 = h1 + size_t(batch) * kPlanHidden;
   return forward(w, x, batch, h1, h2, out, s);
