@@ -218,7 +218,7 @@ static int join_base(int64_t nR, int P, double f, int key_bytes) {
 }
 
 static int64_t join_slice(const pj_plan* plan, int base) {
-  if (plan->probe_slice > 0) return plan->probe_slice;
+  if (plan->probe_slice > 0) return plan->probe_slice < (int64_t(1) << 30) ? plan->probe_slice : int64_t(1) << 30;
   int64_t s = int64_t(base) * 8;
   return s < 16384 ? 16384 : s;
 }

@@ -422,11 +422,11 @@ struct ItemValues {
   const uint32_t* soff;
   uint64_t* out[2];
   uint32_t* chunk_ctr;
-  uint32_t base;
-  uint64_t slice;
+  FastDiv base, slice;  // magic-number division by the build chunk and the probe slice
   __device__ uint64_t value(int, int64_t q) const {
-    const uint64_t nr = roff[q + 1] - roff[q], ns = soff[q + 1] - soff[q];
-    return (nr && ns) ? ((nr + base - 1) / base) * ((ns + slice - 1) / slice) : 0ull;
+    const uint32_t nr = roff[q + 1] - roff[q], ns = soff[q + 1] - soff[q];  // < 2^31
+    if (!nr || !ns) return 0ull;
+    return uint64_t(fastdiv_div(int32_t(nr - 1), base) + 1) * uint64_t(fastdiv_div(int32_t(ns - 1), slice) + 1);
   }
   __device__ void finish(int, int64_t n, uint64_t total) const {
     out[0][n] = total;
@@ -441,8 +441,8 @@ hipError_t launch_join_plan(const Layout& L, int base, int64_t slice, hipStream_
   v.out[0] = L.item_scan;
   v.out[1] = L.item_scan;
   v.chunk_ctr = L.chunk_ctr;
-  v.base = uint32_t(base);
-  v.slice = uint64_t(slice);
+  v.base = fastdiv_make(base);
+  v.slice = fastdiv_make(int32_t(slice));
   return launch_xscan<uint64_t>(v, L.pp.P, 1, reinterpret_cast<uint64_t*>(L.scan_scratch), s);
 }
 

@@ -24,26 +24,46 @@ namespace pj {
 // ---------------------------------------------------------------------------
 // Sketch
 // ---------------------------------------------------------------------------
-// In-LDS bitonic sort (ascending) of N uint64 by a 1024-thread workgroup.
-template <int N>
-__device__ __forceinline__ void bitonic_sort_u64(uint64_t* s) {
-  static_assert(N % 1024 == 0, "N must be a multiple of the workgroup");
-  for (int size = 2; size <= N; size <<= 1) {
-    for (int j = size >> 1; j > 0; j >>= 1) {
-      for (int i = threadIdx.x; i < N; i += 1024) {
-        const int l = i ^ j;
-        if (l > i) {
-          const bool asc = (i & size) == 0;
-          const uint64_t a = s[i], b = s[l];
-          if ((a > b) == asc) {
-            s[i] = b;
-            s[l] = a;
-          }
-        }
+// Bitonic sort (ascending) of PER * 1024 uint64 held PER per thread by a
+// 1024-thread workgroup, element i = PER * tid + c: a compare-exchange partner
+// i ^ j lies in the same thread for j < PER (register swaps), in another lane
+// of the same wave for j < PER * 64 (shuffles), and in another wave otherwise
+// (through LDS, two barriers).  Only the stages whose partners cross waves
+// synchronise the workgroup (10 of 78 for 4096 values).  Ends with the
+// sorted values in v and in s[0 .. PER * 1024).
+template <int PER>
+__device__ __forceinline__ void bitonic_sort_block(uint64_t (&v)[PER], uint64_t* s) {
+  constexpr uint32_t N = uint32_t(PER) * 1024u;
+  const uint32_t tid = threadIdx.x;
+  for (uint32_t size = 2; size <= N; size <<= 1) {
+    for (uint32_t j = size >> 1; j > 0; j >>= 1) {
+      uint64_t w[PER];
+      if (j >= uint32_t(PER) * kWave) {
+#pragma unroll
+        for (int c = 0; c < PER; ++c) s[PER * tid + c] = v[c];
+        __syncthreads();
+#pragma unroll
+        for (int c = 0; c < PER; ++c) w[c] = s[(PER * tid + c) ^ j];
+        __syncthreads();
+      } else if (j >= uint32_t(PER)) {
+#pragma unroll
+        for (int c = 0; c < PER; ++c) w[c] = __shfl_xor(v[c], int(j / PER), kWave);
+      } else {
+#pragma unroll
+        for (int c = 0; c < PER; ++c) w[c] = v[c ^ j];
       }
-      __syncthreads();
+#pragma unroll
+      for (int c = 0; c < PER; ++c) {
+        const uint32_t i = PER * tid + c;
+        const bool keep_min = ((i & j) == 0) == ((i & size) == 0);
+        const uint64_t a = v[c], b = w[c];
+        v[c] = keep_min ? (a < b ? a : b) : (a < b ? b : a);
+      }
     }
   }
+#pragma unroll
+  for (int c = 0; c < PER; ++c) s[PER * tid + c] = v[c];
+  __syncthreads();
 }
 
 template <int KB>
@@ -56,43 +76,52 @@ __global__ __launch_bounds__(1024) void k_sketch(const void* keys, int64_t n, in
   __shared__ float s_red[2 * 1024 / kWave];
   const int tid = threadIdx.x;
   const key_t* k = static_cast<const key_t*>(keys);
+  static_assert(kPlanSample == 4 * 1024 && kPlanBins == 1024, "sort shapes");
   for (int b = tid; b < kPlanBins; b += 1024) s_hist[b] = 0;
   __syncthreads();
-  for (int j = tid; j < kPlanSample; j += 1024) {
-    uint64_t v = 0;
+  uint64_t v[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const int j = 4 * tid + c;
+    v[c] = 0;
     if (n > 0) {
       const int64_t pos = ((2 * int64_t(j) + 1) * n) / (2 * int64_t(kPlanSample));  // < n
       const key_t key = k[pos];
-      v = uint64_t(key);
-      if (KB == 4) v &= 0xffffffffull;
+      v[c] = uint64_t(key);
+      if (KB == 4) v[c] &= 0xffffffffull;
       atomicAdd(&s_hist[murmur3(key) & uint32_t(kPlanBins - 1)], 1u);
     }
-    s_keys[j] = v;
   }
-  __syncthreads();
-  bitonic_sort_u64<kPlanSample>(s_keys);
-  // Runs of equal keys: a run's first position holds its length.  The sort
-  // key (4096 - count) << 12 | position orders runs by count, then by key.
-  for (int i = tid; i < kPlanSample; i += 1024) {
+  bitonic_sort_block<4>(v, s_keys);
+  // Runs of equal keys: a run's first position holds its length (found by a
+  // binary search for the run's end).  The sort key (4096 - count) << 12 |
+  // position orders runs by count, then by key.
+  uint64_t sel[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const int i = 4 * tid + c;
     uint64_t sk = (uint64_t(kPlanSample) << 12) | uint64_t(i);
     if (n > 0 && (i == 0 || s_keys[i] != s_keys[i - 1])) {
-      int e = i + 1;
-      while (e < kPlanSample && s_keys[e] == s_keys[i]) ++e;
-      sk = (uint64_t(kPlanSample - (e - i)) << 12) | uint64_t(i);
+      int lo = i, hi = kPlanSample;  // s_keys[lo] == key, the end is in (lo, hi]
+      const uint64_t key = s_keys[i];
+      while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (s_keys[mid] == key) lo = mid; else hi = mid;
+      }
+      sk = (uint64_t(kPlanSample - (hi - i)) << 12) | uint64_t(i);
     }
-    s_sel[i] = sk;
+    sel[c] = sk;
   }
-  __syncthreads();
   // Sample collision probability: sum over runs of (count / 4096)^2.
   float sq = 0.f;
-  for (int i = tid; i < kPlanSample; i += 1024) {
-    const uint32_t c = uint32_t(kPlanSample) - uint32_t(s_sel[i] >> 12);
-    sq += float(c) * float(c);
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const uint32_t cnt = uint32_t(kPlanSample) - uint32_t(sel[c] >> 12);
+    sq += float(cnt) * float(cnt);
   }
   for (int d = kWave / 2; d > 0; d >>= 1) sq += __shfl_xor(sq, d, kWave);
   if ((tid & (kWave - 1)) == 0) s_red[tid / kWave] = sq;
-  __syncthreads();
-  bitonic_sort_u64<kPlanSample>(s_sel);
+  bitonic_sort_block<4>(sel, s_sel);
   if (tid < kPlanCand) {
     const uint64_t sk = s_sel[tid];
     const uint32_t c = uint32_t(kPlanSample) - uint32_t(sk >> 12);
@@ -103,10 +132,9 @@ __global__ __launch_bounds__(1024) void k_sketch(const void* keys, int64_t n, in
   if (tid == 0) top = float(uint32_t(kPlanSample) - uint32_t(s_sel[0] >> 12));
   __syncthreads();
   // Histogram bins sorted by count, descending (reuse s_keys).
-  for (int b = tid; b < kPlanBins; b += 1024) s_keys[b] = ~uint64_t(s_hist[b]);
-  __syncthreads();
-  bitonic_sort_u64<kPlanBins>(s_keys);
-  for (int b = tid; b < kPlanBins; b += 1024) feat[b] = float(~s_keys[b]) / float(kPlanSample);
+  uint64_t hb[1] = {~uint64_t(s_hist[tid])};
+  bitonic_sort_block<1>(hb, s_keys);
+  feat[tid] = float(~hb[0]) / float(kPlanSample);
   if (tid == 0) {
     float tot = 0.f;
     for (int w = 0; w < 1024 / kWave; ++w) tot += s_red[w];
