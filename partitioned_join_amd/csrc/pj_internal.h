@@ -77,9 +77,8 @@ struct JoinArgs {
   pj_pair* spill;          // kJoinGrid * kOutChunk pairs
   uint32_t* chunk_ctr;     // claimed output chunks
   uint64_t* wg_last;       // [grid] (chunk << 32 | fill) or ~0 when no chunk; fill <= kOutChunk
-  int table_cap;           // LDS slots available for the build table (a multiple of 4)
   int bucket_slots;        // keys per 16-byte bucket: 4 (4-byte keys) or 2 (8-byte keys)
-  uint32_t cap_buckets;    // largest power-of-two bucket count that fits table_cap
+  uint32_t cap_buckets;    // largest power-of-two bucket count within join_table_capacity() slots
   uint32_t nr, ns;         // table sizes (bounds for the item ranges)
   const uint4* desc;       // expanded work item descriptors (2 uint4 per item)
   uint64_t desc_cap;       // items with a descriptor; later items are decoded by search
@@ -146,8 +145,9 @@ hipError_t launch_join_generic(const void* rt, const uint32_t* roff, const void*
                                int64_t capacity, int64_t* d_count, char* ws, hipStream_t s);
 size_t join_generic_ws_bytes(int B);
 
-// LDS hash table slots of one join item (the stage of one probe round takes
-// the rest of kJoinLdsBytes); the build chunk is at most half of it.
+// Build-table slots that kJoinLdsBytes leaves next to the stage of one probe
+// round; a build chunk is at most half of it.  The table itself has the
+// largest power-of-two bucket count within it (cap_buckets).
 int join_table_capacity(int key_bytes);
 
 // log2 of the bucket count of the table of a build chunk of c tuples: the

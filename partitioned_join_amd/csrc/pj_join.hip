@@ -9,11 +9,13 @@
 //   k_join       persistent workgroups; per item: LDS open-addressing table of
 //                a power-of-two number of 16-byte key buckets with at least
 //                4*|chunk| slots (occupancy <= 1/4), bucket = top bits of a
-//                multiplicative hash of the key, linear probing with LDS CAS
-//                from the bucket's first slot (the reference hashes with
-//                MurmurHash3 and slots with int_fastdiv,
-//                src/hash_join_kernels.cuh:84-129; the slot never changes the
-//                result), then the probe slice walks the table (:134-181).  Slots fill each bucket in
+//                multiplicative hash of the key.  A build tuple claims the next
+//                slot of its bucket with one LDS atomic add on the bucket's
+//                claim counter, moving on to the following bucket when that one
+//                is full (the reference hashes with MurmurHash3, slots with
+//                int_fastdiv and inserts by CAS, src/hash_join_kernels.cuh:84-129;
+//                the slot never changes the result), then the probe slice walks
+//                the table (:134-181).  Slots fill each bucket in
 //                order, so a probe reads one whole bucket per step (one 16-byte
 //                LDS read) and stops at a match or at an empty slot: nearly
 //                every probe ends in its first bucket.  Probes of one thread
@@ -52,9 +54,24 @@ static_assert(4 * kJoinBlock >= table_cap_of(8) / 2, "NB = 4 covers the largest 
 
 int join_table_capacity(int key_bytes) { return table_cap_of(key_bytes); }
 
-static size_t join_lds_bytes(int key_bytes, int table_slots) {
-  return size_t(table_slots) * join_slot_bytes(key_bytes) + size_t(join_stage_pairs(key_bytes)) * 8;
+// The largest power-of-two bucket count whose slots fit table_cap: the table
+// of a chunk never has more buckets.
+static constexpr uint32_t cap_buckets_of(int key_bytes) {
+  const uint32_t b = uint32_t(table_cap_of(key_bytes) / (16 / key_bytes));
+  uint32_t p = 1;
+  while (2 * p <= b) p *= 2;
+  return p;
 }
+
+// LDS of a join workgroup: keys and payloads of cap_buckets buckets, one
+// claim counter per bucket, then the output stage.
+static size_t join_lds_bytes(int key_bytes) {
+  const size_t slots = size_t(cap_buckets_of(key_bytes)) * size_t(16 / key_bytes);
+  return slots * join_slot_bytes(key_bytes) + size_t(cap_buckets_of(key_bytes)) * 4 +
+         size_t(join_stage_pairs(key_bytes)) * 8;
+}
+static_assert(2 * (4096 * 8 + 1024 * 4 + kJoinProbe4 * kJoinBlock * 8) <= 160 * 1024, "two 4-byte-key workgroups per CU");
+static_assert(2 * (4096 * 12 + 2048 * 4 + kJoinProbe8 * kJoinBlock * 8) <= 160 * 1024, "two 8-byte-key workgroups per CU");
 
 // ---------------------------------------------------------------------------
 // Output staging shared by the join workgroup.
@@ -115,6 +132,10 @@ __device__ __forceinline__ void copy_stage(const OutSink o, StageCtl* ctl, const
   const uint32_t n = ctl->copy_n, a = ctl->a;
   const uint64_t d0 = ctl->dst0, d1 = ctl->dst1;
   if (threadIdx.x == 0) ctl->n = 0;
+  if (d0 + a <= o.capacity && (n == a || d1 + (n - a) <= o.capacity)) {  // uniform: nothing goes to the spill area
+    for (uint32_t i = threadIdx.x; i < n; i += BLOCK) o.out[i < a ? d0 + i : d1 + (i - a)] = s_stage[i];
+    return;
+  }
   for (uint32_t i = threadIdx.x; i < n; i += BLOCK) {
     const uint64_t v = i < a ? d0 + i : d1 + (i - a);
     store_pair(o, v, s_stage[i]);
@@ -129,12 +150,6 @@ __device__ __forceinline__ void flush_stage(const OutSink o, StageCtl* ctl, cons
   lds_barrier();
   copy_stage<BLOCK>(o, ctl, s_stage);
   lds_barrier();
-}
-
-__device__ __forceinline__ uint32_t lds_cas(uint32_t* p, uint32_t cmp, uint32_t val) { return atomicCAS(p, cmp, val); }
-__device__ __forceinline__ uint64_t lds_cas(uint64_t* p, uint64_t cmp, uint64_t val) {
-  return uint64_t(atomicCAS(reinterpret_cast<unsigned long long*>(p), (unsigned long long)cmp,
-                            (unsigned long long)val));
 }
 
 // The keys of one 16-byte bucket.
@@ -279,9 +294,11 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
   constexpr uint32_t BS = 16 / sizeof(key_t);  // slots per bucket
   static_assert(STAGE >= uint32_t(R) * kWave, "one wave's matches of a step must fit an empty stage");
   extern __shared__ __align__(16) uint8_t s_raw[];
-  key_t* s_keys = reinterpret_cast<key_t*>(s_raw);                                       // table_cap keys
-  int32_t* s_pays = reinterpret_cast<int32_t*>(s_raw + size_t(A.table_cap) * sizeof(key_t));  // their payloads
-  uint2* s_stage = reinterpret_cast<uint2*>(s_raw + size_t(A.table_cap) * (sizeof(key_t) + 4));  // STAGE pairs
+  const uint32_t cap_slots = A.cap_buckets * BS;
+  key_t* s_keys = reinterpret_cast<key_t*>(s_raw);                                        // cap_slots keys
+  int32_t* s_pays = reinterpret_cast<int32_t*>(s_raw + size_t(cap_slots) * sizeof(key_t));  // their payloads
+  uint32_t* s_cnt = reinterpret_cast<uint32_t*>(s_pays + cap_slots);                      // claims per bucket
+  uint2* s_stage = reinterpret_cast<uint2*>(s_cnt + A.cap_buckets);                       // STAGE pairs
   __shared__ StageCtl ctl;
   __shared__ uint32_t s_any[3];
   // sentinel search (caller-partitioned tuples): chunk minimum and maximum,
@@ -405,42 +422,73 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
     {
       const uint4 fill = bucket_fill(sentinel);
       uint4* s4 = reinterpret_cast<uint4*>(s_keys);
-      for (uint32_t i = tid; i < tsize / BS; i += BLOCK) s4[i] = fill;
+      for (uint32_t i = tid; i < tsize / BS; i += BLOCK) {
+        s4[i] = fill;
+        s_cnt[i] = 0u;
+      }
     }
     lds_barrier();
     // The previous item's matches go out now, after this item's prefetch was
     // issued: the in-order memory counter that the next wait for the prefetch
     // uses then finds these stores long complete.
     copy_stage<BLOCK>(sink, &ctl, s_stage);
-    // Build: every thread's tuples claim slots together, one CAS each per step.
-    // A CAS that finds the tuple's own key reveals a duplicate build key in
-    // the chunk (two equal keys start at the same slot, so the later one
-    // always meets the earlier one or loses the slot to it).
+    // Build: each tuple takes the next slot of its bucket from the bucket's
+    // claim counter (one LDS atomic add), moving on to the following bucket
+    // when its claim is past the bucket's end.  The claims of one bucket are
+    // consecutive, so its filled slots are a prefix of it, and every bucket a
+    // key passed over is full: a probe that walks from the key's bucket to
+    // the first bucket with an empty slot meets every copy of the key.
     bool dup = false;
     {
-      uint32_t bslot[NB];
+      const uint32_t bmask = (1u << d.lg) - 1u;
+      uint32_t bkt[NB], home[NB];
       uint32_t pend = 0;
 #pragma unroll
       for (int k = 0; k < NB; ++k) {
         const bool v = d.rlo + uint32_t(k) * BLOCK + tid < d.rhi;
         pend |= v ? (1u << k) : 0u;
-        bslot[k] = v ? first_slot<BS>(KTr::key(bt[k]), d.lg) : 0u;
+        bkt[k] = v ? first_slot<BS>(KTr::key(bt[k]), d.lg) / BS : 0u;
+        home[k] = bkt[k];
       }
+      const uint32_t valid = pend;
       while (pend) {
-        key_t old[NB];
+        uint32_t pos[NB];
 #pragma unroll
         for (int k = 0; k < NB; ++k)
-          if (pend & (1u << k)) old[k] = lds_cas(&s_keys[bslot[k]], sentinel, KTr::key(bt[k]));
+          if (pend & (1u << k)) pos[k] = atomicAdd(&s_cnt[bkt[k]], 1u);
 #pragma unroll
         for (int k = 0; k < NB; ++k)
           if (pend & (1u << k)) {
-            if (old[k] == sentinel) {
-              s_pays[bslot[k]] = KTr::pay(bt[k]);
+            if (pos[k] < BS) {
+              const uint32_t sl = bkt[k] * BS + pos[k];
+              s_keys[sl] = KTr::key(bt[k]);
+              s_pays[sl] = KTr::pay(bt[k]);
               pend &= ~(1u << k);
             } else {
-              dup |= old[k] == KTr::key(bt[k]);
-              bslot[k] = (bslot[k] + 1u) & tmask;
+              bkt[k] = (bkt[k] + 1u) & bmask;
             }
+          }
+      }
+      if (!UNIQUE) {
+        // Duplicate keys in the chunk: each tuple counts its key over the
+        // buckets from its own bucket to the one it landed in.  Equal keys
+        // share their first bucket, so the copy that landed furthest along
+        // counts every copy, itself included.
+        lds_barrier();
+#pragma unroll
+        for (int k = 0; k < NB; ++k)
+          if (valid & (1u << k)) {
+            const key_t key = KTr::key(bt[k]);
+            uint32_t b = home[k], n = 0;
+            while (true) {
+              key_t kk[BS];
+              bucket_keys(*reinterpret_cast<const uint4*>(&s_keys[b * BS]), kk);
+#pragma unroll
+              for (int i = 0; i < int(BS); ++i) n += kk[i] == key ? 1u : 0u;
+              if (b == bkt[k]) break;
+              b = (b + 1u) & bmask;
+            }
+            dup |= n > 1u;
           }
       }
     }
@@ -463,6 +511,38 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
         if constexpr (!DIGIT_SENTINEL) v = v && KTr::key(ptr[k]) != sentinel;
         walk |= v ? (1u << k) : 0u;
         slot[k] = v ? first_slot<BS>(KTr::key(ptr[k]), d.lg) : 0u;
+      }
+      // The first step, taken by nearly every probe, without branches: the
+      // bucket reads go out in groups of G (an invalid probe reads bucket 0
+      // and is ignored) and are examined with selects.  Later steps, taken by
+      // few probes, branch per probe so that finished probes read nothing.
+      {
+        constexpr int G = 3;
+#pragma unroll
+        for (int g = 0; g < R; g += G) {
+          uint4 bq[G];
+#pragma unroll
+          for (int j = 0; j < G; ++j)
+            if (g + j < R) bq[j] = *reinterpret_cast<const uint4*>(&s_keys[slot[g + j]]);
+#pragma unroll
+          for (int j = 0; j < G; ++j)
+            if (g + j < R) {
+              const int k = g + j;
+              key_t kk[BS];
+              bucket_keys(bq[j], kk);
+              const key_t key = KTr::key(ptr[k]);
+              const bool active = ((walk >> k) & 1u) != 0u;
+              const bool empty = kk[BS - 1] == sentinel;
+              uint32_t hit = BS;
+#pragma unroll
+              for (int i = BS - 1; i >= 0; --i) hit = kk[i] == key ? uint32_t(i) : hit;
+              const bool is_hit = active && hit < BS;
+              const bool cont = active && hit >= BS && !empty;
+              found |= is_hit ? (1u << k) : 0u;
+              walk &= cont ? ~0u : ~(1u << k);
+              slot[k] = is_hit ? slot[k] + hit : (cont ? ((slot[k] + BS) & tmask) : slot[k]);
+            }
+        }
       }
       while (__ballot(walk != 0)) {
 #pragma unroll
@@ -778,7 +858,7 @@ static void launch_join_kernel_t(const JoinArgs& A, int key_bytes, int base, int
 // partition q's keys all have low bits q (see k_join).
 static hipError_t launch_join_kernel(const JoinArgs& A, int key_bytes, int base, int unique, bool digit_sentinel,
                                      hipStream_t s) {
-  const size_t lds = join_lds_bytes(key_bytes, A.table_cap);
+  const size_t lds = join_lds_bytes(key_bytes);
   if (digit_sentinel) launch_join_kernel_t<true>(A, key_bytes, base, unique, lds, s);
   else launch_join_kernel_t<false>(A, key_bytes, base, unique, lds, s);
   return hipGetLastError();
@@ -804,9 +884,8 @@ static JoinArgs join_args(const Layout& L, int key_bytes, int base, int64_t slic
   A.ns = L.t[1].n;
   A.desc = reinterpret_cast<const uint4*>(L.desc);
   A.desc_cap = L.desc_cap;
-  A.table_cap = join_table_capacity(key_bytes);
   A.bucket_slots = 16 / key_bytes;
-  A.cap_buckets = 1u << (31 - __builtin_clz(uint32_t(A.table_cap / A.bucket_slots)));
+  A.cap_buckets = cap_buckets_of(key_bytes);
   return A;
 }
 

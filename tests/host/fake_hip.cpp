@@ -165,11 +165,12 @@ void check_join_args(const char* k, const JoinArgs& A, uint32_t grid) {
   need(k, "chunk_ctr", A.chunk_ctr, 4);
   need(k, "wg_last", A.wg_last, uint64_t(grid) * 8);
   need(k, "desc", A.desc, A.desc_cap * 32);
-  if (A.base < 1 || A.base > A.table_cap / 2 - 1) err("%s: base %d outside [1, %d]", k, A.base, A.table_cap / 2 - 1);
+  const int table_cap = pj::join_table_capacity(kb);
+  if (A.base < 1 || A.base > table_cap / 2 - 1) err("%s: base %d outside [1, %d]", k, A.base, table_cap / 2 - 1);
   // power-of-two bucket tables: within the LDS slots, and more slots than the largest chunk
   const uint32_t cb = A.cap_buckets, bs = uint32_t(A.bucket_slots);
-  if (cb == 0 || (cb & (cb - 1)) != 0 || cb * bs > uint32_t(A.table_cap) || cb * bs <= uint32_t(A.base))
-    err("%s: cap_buckets %u (x %u slots) against table_cap %d and base %d", k, cb, bs, A.table_cap, A.base);
+  if (cb == 0 || (cb & (cb - 1)) != 0 || cb * bs > uint32_t(table_cap) || cb * bs <= uint32_t(A.base))
+    err("%s: cap_buckets %u (x %u slots) against table_cap %d and base %d", k, cb, bs, table_cap, A.base);
   if (A.slice < 1) err("%s: probe slice %lld < 1", k, (long long)A.slice);
 }
 
@@ -231,6 +232,13 @@ void check_launch(const Launch& L) {
     need(k, "soff", A.soff, uint64_t(A.P + 1) * 4);
   } else if (starts(name, "pj::k_join<")) {
     check_join_args(k, *static_cast<const JoinArgs*>(a0), L.grid.x);
+    // keys, payloads and a claim counter per bucket, plus the stage; two workgroups share a CU
+    const JoinArgs& A = *static_cast<const JoinArgs*>(a0);
+    const uint64_t kb = A.bucket_slots == 4 ? 4 : 8;
+    const uint64_t stage = uint64_t(tmpl_int(name, 3)) * uint64_t(L.block.x) * 8;
+    const uint64_t want = uint64_t(A.cap_buckets) * (uint64_t(A.bucket_slots) * (kb + 4) + 4) + stage;
+    if (L.shmem < want) err("%s: %zu bytes of LDS, the table and stage need %llu", k, L.shmem, (unsigned long long)want);
+    if (L.shmem > 80 * 1024) err("%s: %zu bytes of LDS leave no room for a second workgroup per CU", k, L.shmem);
   }
 }
 
