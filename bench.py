@@ -5,8 +5,9 @@ LDS hash join, output compaction) with inputs resident in HBM.
 
 Prints one JSON line: whole-job tuples/s, the roofline of the dominant kernel
 (algorithmic bytes / its hipEvent-timed duration vs 8 TB/s), per-phase times,
-the plain and learned plans on the Zipf(1.0) skewed join, the batched-joins
-record, and a CPU baseline: the oracle's restatement of the reference's
+the plain and learned plans on the Zipf(1.0) skewed join and on eight seeded
+joins the planner was not trained on, the batched-joins record, and a CPU
+baseline: the oracle's restatement of the reference's
 inner_join_cpu (std::unordered_map, serial build, OpenMP probe; the
 reference's own CPU code needs the CUDA toolkit and is not built here) timed
 on this host's cores on the full 64M x 256M workload, the 1M x 4M one and the
@@ -14,6 +15,7 @@ on this host's cores on the full 64M x 256M workload, the 1M x 4M one and the
 """
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -214,6 +216,52 @@ def variant_records(pj, torch, steps):
     return recs
 
 
+def learned_held_out(pj, torch, planner, w, joins=8, first=2000):
+    """The learned planner on joins it was not trained on: the first `joins`
+    seeded joins from index `first` on (training used indices below it) with at
+    most 16M build and 128M probe rows.  Each is timed with the plain plan and,
+    when it differs, with the learned plan (same pair count asserted); the
+    summary is the geometric mean of plain / learned time with the plan reused
+    and with planning included."""
+    rows = []
+    i = first
+    while len(rows) < joins:
+        sp = planner.join_spec(i)
+        i += 1
+        if sp["n_build"] > (16 << 20) or sp["n_probe"] > (128 << 20):
+            continue
+        bk, pk = planner.make_join(sp)
+        torch.cuda.synchronize()
+        plain = pj.plan_default(bk.numel(), pk.numel(), bk.element_size())
+        planner.plan_learned(bk, pk, w)
+        t0 = time.perf_counter()
+        lp, fb = planner.plan_learned(bk, pk, w)
+        plan_ms = (time.perf_counter() - t0) * 1000.0
+        cap = pk.numel() + 1024
+        pms, _, pc = run_join_steps(pj, torch, bk, pk, plain, 3, 1, n_cap=cap)
+        same = (lp.passes, lp.bits1, lp.bits2) == (plain.passes, plain.bits1, plain.bits2)
+        lms = pms
+        if not same:
+            lms, _, lc = run_join_steps(pj, torch, bk, pk, lp, 3, 1, n_cap=cap)
+            assert lc == pc, (lc, pc)
+        rows.append({"join": sp["i"], "build_rows": sp["n_build"], "probe_rows": sp["n_probe"],
+                     "key_bytes": sp["key_bytes"], "zipf_theta": round(sp["theta"], 3),
+                     "match": round(sp["match"], 3), "plain_plan": [plain.passes, plain.bits1, plain.bits2],
+                     "learned_plan": [lp.passes, lp.bits1, lp.bits2], "fallback_to_plain": fb,
+                     "plain_ms": pms, "learned_ms": lms, "planning_ms": plan_ms})
+        del bk, pk
+        torch.cuda.empty_cache()
+    reuse = [r["plain_ms"] / r["learned_ms"] for r in rows]
+    incl = [r["plain_ms"] / (r["learned_ms"] + r["planning_ms"]) for r in rows]
+    gm = lambda v: math.exp(sum(math.log(x) for x in v) / len(v))
+    return {"workload": f"seeded held-out joins {first}.. (build <= 16M, probe <= 128M rows, int32 or int64 keys, "
+                        "Zipf probes with misses)", "joins": rows,
+            "plans_differing": sum(r["plain_plan"] != r["learned_plan"] for r in rows),
+            "learned_faster": sum(r["learned_ms"] < r["plain_ms"] for r in rows),
+            "learned_slower": sum(r["learned_ms"] > r["plain_ms"] for r in rows),
+            "speedup_plan_reused_geomean": gm(reuse), "speedup_planning_included_geomean": gm(incl)}
+
+
 def log(msg):
     """Progress line on stderr (the JSON result is the only stdout line)."""
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
@@ -352,6 +400,8 @@ def main():
                         "learned_plan_equals_plain": [lplan.passes, lplan.bits1, lplan.bits2] ==
                                                      [zplan.passes, zplan.bits1, zplan.bits2],
                         "learned_pair_count_equal_plain": lcount == zcount})
+            line["learned_held_out"] = learned_held_out(pj, torch, planner, w)
+            log("learned planner on held-out joins done")
         line["zipf_theta1"] = rec
         log(f"Zipf(1.0) join: plain {zms:.3f} ms")
         c3_host = (zb.cpu().numpy(), zp.cpu().numpy()) if not args.no_cpu else None

@@ -328,9 +328,12 @@ def build_dataset(first: int, count: int, out_prefix: str, device="cuda", log=pr
 
 
 # --------------------------------------------------------------------------- training
-def train(X, Y, H, epochs: int = 60, batch: int = 64, lr: float = 1e-3, seed: int = 0, device="cuda", log=print):
+def train(X, Y, H, epochs: int = 60, batch: int = 64, lr: float = 1e-3, seed: int = 0, device="cuda", log=print,
+          init=None):
+    """Adam over the labelled joins; from He-initialised weights, or from a
+    copy of `init` (fine-tuning)."""
     torch = _t()
-    w = init_weights(seed, device)
+    w = init_weights(seed, device) if init is None else init.detach().clone().float().to(device)
     m = torch.zeros_like(w)
     v = torch.zeros_like(w)
     Xd = torch.from_numpy(np.ascontiguousarray(X, dtype=np.float32)).to(device)
@@ -354,10 +357,12 @@ def train(X, Y, H, epochs: int = 60, batch: int = 64, lr: float = 1e-3, seed: in
 
 def evaluate(w, X, Y, H, C, E, recs, device="cuda"):
     """Plan accuracy, fallback rate and time regret (chosen plan time / best
-    plan time) of the planner on labelled joins."""
+    plan time) of the planner on labelled joins, and the plain plan's regret
+    on the joins where both plans were among the timed candidates."""
+    from . import plan_default
     torch = _t()
     logits = forward(w, torch.from_numpy(np.ascontiguousarray(X, dtype=np.float32)).to(device)).cpu().numpy()
-    hits, fallbacks, regrets = 0, 0, []
+    hits, fallbacks, regrets, paired, wins, losses = 0, 0, [], [], 0, 0
     for i, rec in enumerate(recs):
         sp = rec["spec"]
         plan, fb = decode(logits[i], C[i], E[i], sp["n_build"], sp["n_probe"], sp["key_bytes"])
@@ -367,11 +372,21 @@ def evaluate(w, X, Y, H, C, E, recs, device="cuda"):
         t = rec["timings"].get(key)
         if t is not None:
             regrets.append(t / best)
+        plain = plan_default(sp["n_build"], sp["n_probe"], sp["key_bytes"])
+        tp = rec["timings"].get(f"{plain.passes}:{plain.bits1}:{plain.bits2}")
+        if t is not None and tp is not None:
+            paired.append((t / best, tp / best))
+            wins += int(t < tp)
+            losses += int(t > tp)
         hits += int(list(rec["best"]) == [plan.passes, plan.bits1, plan.bits2])
     n = len(recs)
+    pr = np.array(paired).reshape(-1, 2)
     return {"joins": n, "plan_accuracy": hits / n, "fallback_rate": fallbacks / n,
             "mean_regret": float(np.mean(regrets)) if regrets else None,
-            "timed_choices": len(regrets)}
+            "timed_choices": len(regrets),
+            "paired_joins": len(pr), "paired_mean_regret": float(pr[:, 0].mean()) if len(pr) else None,
+            "plain_mean_regret": float(pr[:, 1].mean()) if len(pr) else None,
+            "learned_faster": wins, "learned_slower": losses}
 
 
 def main(argv: Optional[list] = None):
@@ -387,6 +402,13 @@ def main(argv: Optional[list] = None):
     b.add_argument("--epochs", type=int, default=60)
     b.add_argument("--holdout", type=float, default=0.2)
     b.add_argument("--weights", default=WEIGHTS_PATH, help="where to write the trained weights")
+    b.add_argument("--lr", type=float, default=1e-3)
+    b.add_argument("--seed", type=int, default=0)
+    b.add_argument("--init", default=None, help="weights to fine-tune instead of a fresh initialisation")
+    e = sub.add_parser("evaluate", help="score weights on labelled joins against the plain plan")
+    e.add_argument("--data", nargs="+", required=True, help=".npz/.json prefixes from `label`")
+    e.add_argument("--holdout", type=float, default=0.2)
+    e.add_argument("--weights", default=WEIGHTS_PATH)
     args = ap.parse_args(argv)
     if args.cmd == "label":
         build_dataset(args.first, args.count, args.out, log=lambda s: print(s, flush=True))
@@ -403,13 +425,22 @@ def main(argv: Optional[list] = None):
     n_test = int(n * args.holdout)
     tr = slice(0, n - n_test)
     te = slice(n - n_test, n)
-    w = train(X[tr], Y[tr], H[tr], epochs=args.epochs, log=lambda s: print(s, flush=True))
+    if args.cmd == "evaluate":
+        w = load_weights(args.weights)
+        print(json.dumps({"weights": args.weights,
+                          "train": evaluate(w, X[tr], Y[tr], H[tr], C[tr], E[tr], recs[tr]),
+                          "holdout": evaluate(w, X[te], Y[te], H[te], C[te], E[te], recs[te])}), flush=True)
+        return
+    init = load_weights(args.init) if args.init else None
+    w = train(X[tr], Y[tr], H[tr], epochs=args.epochs, lr=args.lr, seed=args.seed, init=init,
+              log=lambda s: print(s, flush=True))
     ev_tr = evaluate(w, X[tr], Y[tr], H[tr], C[tr], E[tr], recs[tr])
     ev_te = evaluate(w, X[te], Y[te], H[te], C[te], E[te], recs[te])
     print(json.dumps({"train": ev_tr, "holdout": ev_te}), flush=True)
     meta_path = os.path.splitext(args.weights)[0] + ".json"
     save_weights(w, {"features": FEATURES, "hidden": HIDDEN, "logits": LOGITS, "train": ev_tr, "holdout": ev_te,
-                     "joins": n, "epochs": args.epochs,
+                     "joins": n, "epochs": args.epochs, "lr": args.lr, "seed": args.seed,
+                     "fine_tuned_from": os.path.basename(args.init) if args.init else None,
                      "data": "seeded training joins i = 0..n-1 of planner.join_spec"}, args.weights, meta_path)
 
 
