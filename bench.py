@@ -189,6 +189,7 @@ def variant_records(pj, torch, steps):
     bp = torch.randint(-2**31, 2**31 - 1, (n_r,), generator=g, device="cuda", dtype=torch.int32)
     pp = torch.randint(-2**31, 2**31 - 1, (n_s,), generator=g, device="cuda", dtype=torch.int32)
     plan = pj.plan_default(n_r, n_s, 4)
+    plan.unique_build_keys = 1  # the reference's payload benchmark passes -u -V
     ms, ph, count = run_join_steps(pj, torch, bk, pk, plan, steps, 1, bp, pp)
     ab = algorithmic_bytes(n_r, n_s, count, plan.passes, w=8)
     recs["payload_columns"] = {"workload": "64M x 256M int32 (the headline keys) with int32 payload columns on both "
@@ -336,8 +337,16 @@ def main():
     bk, pk = G.workload_config2()
     torch.cuda.synchronize()
     plan = pj.plan_default(bk.numel(), pk.numel())
+    # The build keys are unique, and the plan says so, as the reference's own
+    # benchmark does (-u, scripts/bench_partitioned_join.sh:19): a probe stops
+    # at its first match and the build skips its duplicate check.
+    plan.unique_build_keys = 1
     ms, ph, count = run_join_steps(pj, torch, bk, pk, plan, args.steps, args.warmup)
     log(f"64M x 256M join: {ms:.3f} ms per step, {count} pairs")
+    if not args.no_variants:  # the same join without the promise: the duplicate check runs
+        plan_nu = pj.plan_default(bk.numel(), pk.numel())
+        ms_nu, ph_nu, count_nu = run_join_steps(pj, torch, bk, pk, plan_nu, max(3, args.steps // 2), 1)
+        assert count_nu == count, (count_nu, count)
     n_r, n_s = bk.numel(), pk.numel()
     ab = algorithmic_bytes(n_r, n_s, count, plan.passes)
     kern = {"pass1": ph["pass1_ms"], "pass2": ph["pass2_ms"], "join": ph["join_ms"],
@@ -360,7 +369,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": "radix join, 64M unique int32 build keys (perm of [0,2^26)) x 256M int32 probe "
                                "keys, 50% match (rest in [2^26,2^27)), row-id payloads, output (build_idx, "
-                               "probe_idx) pairs", "build_rows": n_r, "probe_rows": n_s, "pairs": count,
+                               "probe_idx) pairs; the plan declares the build keys unique (the reference "
+                               "benchmark's -u)", "build_rows": n_r, "probe_rows": n_s, "pairs": count,
                    "plan": plan.as_dict()},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dom), "kernel": dom,
@@ -369,6 +379,8 @@ def main():
         "whole_job_roofline_frac": ab["total"] / (ms / 1000.0) / 1e9 / HBM_PEAK_GBS,
         "algorithmic_bytes_total": ab["total"],
     }
+    if not args.no_variants:
+        line["without_unique_promise"] = {"ms": ms_nu, "join_ms": ph_nu["join_ms"], "plan": plan_nu.as_dict()}
     c2_host = (bk.cpu().numpy(), pk.cpu().numpy()) if not args.no_cpu else None
     del bk, pk
     torch.cuda.empty_cache()
@@ -377,6 +389,7 @@ def main():
     if not args.no_skew:
         zb, zp = G.workload_config3(1.0)
         zplan = pj.plan_default(zb.numel(), zp.numel())
+        zplan.unique_build_keys = 1  # unique build keys, as the reference sets for its probe-skew mode (main.cu:346)
         zms, zph, zcount = run_join_steps(pj, torch, zb, zp, zplan, max(3, args.steps // 2), 1)
         rec = {"workload": "16M unique int32 build x 256M Zipf(1.0) probes", "pairs": zcount,
                "plain_tuples_per_s": (zb.numel() + zp.numel()) / (zms / 1000.0), "plain_ms": zms,
@@ -389,6 +402,7 @@ def main():
             for _ in range(3):
                 lplan, fallback = planner.plan_learned(zb, zp, w)
             plan_ms = (time.perf_counter() - t0) * 1000.0 / 3
+            lplan.unique_build_keys = 1
             lms, lph, lcount = run_join_steps(pj, torch, zb, zp, lplan, max(3, args.steps // 2), 1)
             assert lcount == zcount, (lcount, zcount)
             zt = zb.numel() + zp.numel()

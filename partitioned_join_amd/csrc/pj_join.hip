@@ -37,6 +37,7 @@
 #include "pj_device.h"
 #include "pj_internal.h"
 
+
 namespace pj {
 
 static int join_probe_per_thread(int key_bytes) { return key_bytes == 4 ? kJoinProbe4 : kJoinProbe8; }
@@ -331,17 +332,27 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
   raw_desc_load(A, item + gridDim.x, ra, rb);
   ItemDesc dn = desc_from_raw(A, ra, rb, item + gridDim.x, total_items);
   tup_t bt[NB], pt[R];
+  // Row k (tuples lo + k * BLOCK + tid) loads only if it starts below the
+  // range's end (a uniform branch: a row wholly past it costs no memory
+  // instruction) and then with indices clamped to the last tuple, without a
+  // per-lane branch; lanes past the end reread that tuple and are excluded
+  // by count (lanes_of) wherever used.  Rows past the end hold zeros.
   auto load_item = [&](const ItemDesc& x, tup_t (&b)[NB], tup_t (&p)[R]) {
 #pragma unroll
     for (int k = 0; k < NB; ++k) {
-      const uint32_t i = x.rlo + uint32_t(k) * BLOCK + tid;
-      if (i < x.rhi) b[k] = rt[i];
+      if (x.rlo + uint32_t(k) * BLOCK < x.rhi) b[k] = rt[min(x.rlo + uint32_t(k) * BLOCK + tid, x.rhi - 1u)];
+      else b[k] = tup_t{};
     }
 #pragma unroll
     for (int k = 0; k < R; ++k) {
-      const uint32_t j = x.slo + uint32_t(k) * BLOCK + tid;
-      if (j < x.shi) p[k] = st[j];
+      if (x.slo + uint32_t(k) * BLOCK < x.shi) p[k] = st[min(x.slo + uint32_t(k) * BLOCK + tid, x.shi - 1u)];
+      else p[k] = tup_t{};
     }
+  };
+  // Bit k set: tuple lo + k * BLOCK + tid lies below hi (k < K).
+  auto lanes_of = [&](uint32_t lo, uint32_t hi, int K) -> uint32_t {
+    const uint32_t n = hi > lo + uint32_t(tid) ? (hi - lo - uint32_t(tid) + BLOCK - 1u) / BLOCK : 0u;
+    return n >= uint32_t(K) ? (1u << K) - 1u : (1u << n) - 1u;
   };
   if (d.valid) load_item(d, bt, pt);
   // A value absent from the item's build chunk, for caller-partitioned
@@ -441,15 +452,10 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
     bool dup = false;
     {
       const uint32_t bmask = (1u << d.lg) - 1u;
-      uint32_t bkt[NB], home[NB];
-      uint32_t pend = 0;
+      uint32_t bkt[NB];
+      uint32_t pend = lanes_of(d.rlo, d.rhi, NB);
 #pragma unroll
-      for (int k = 0; k < NB; ++k) {
-        const bool v = d.rlo + uint32_t(k) * BLOCK + tid < d.rhi;
-        pend |= v ? (1u << k) : 0u;
-        bkt[k] = v ? first_slot<BS>(KTr::key(bt[k]), d.lg) / BS : 0u;
-        home[k] = bkt[k];
-      }
+      for (int k = 0; k < NB; ++k) bkt[k] = first_slot<BS>(KTr::key(bt[k]), d.lg) / BS;
       const uint32_t valid = pend;
       while (pend) {
         uint32_t pos[NB];
@@ -473,23 +479,51 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
         // Duplicate keys in the chunk: each tuple counts its key over the
         // buckets from its own bucket to the one it landed in.  Equal keys
         // share their first bucket, so the copy that landed furthest along
-        // counts every copy, itself included.
+        // counts every copy, itself included.  The home buckets are read
+        // together without branches (a tuple's home bucket is recomputed from
+        // its key rather than kept in a register); the few tuples that landed
+        // further on walk on one bucket at a time.
         lds_barrier();
+        constexpr int G = 3;
+        uint32_t far = 0;
 #pragma unroll
-        for (int k = 0; k < NB; ++k)
-          if (valid & (1u << k)) {
-            const key_t key = KTr::key(bt[k]);
-            uint32_t b = home[k], n = 0;
-            while (true) {
+        for (int g = 0; g < NB; g += G) {
+          uint4 hq[G];
+#pragma unroll
+          for (int j = 0; j < G; ++j)
+            if (g + j < NB) hq[j] = *reinterpret_cast<const uint4*>(&s_keys[first_slot<BS>(KTr::key(bt[g + j]), d.lg)]);
+#pragma unroll
+          for (int j = 0; j < G; ++j)
+            if (g + j < NB) {
+              const int k = g + j;
               key_t kk[BS];
-              bucket_keys(*reinterpret_cast<const uint4*>(&s_keys[b * BS]), kk);
+              bucket_keys(hq[j], kk);
+              const key_t key = KTr::key(bt[k]);
+              uint32_t n = 0;
 #pragma unroll
               for (int i = 0; i < int(BS); ++i) n += kk[i] == key ? 1u : 0u;
-              if (b == bkt[k]) break;
-              b = (b + 1u) & bmask;
+              const bool v = (valid >> k) & 1u;
+              dup |= v && n > 1u;
+              far |= v && bkt[k] != first_slot<BS>(key, d.lg) / BS ? (1u << k) : 0u;
             }
-            dup |= n > 1u;
-          }
+        }
+        if (far) {
+#pragma unroll
+          for (int k = 0; k < NB; ++k)
+            if (far & (1u << k)) {
+              const key_t key = KTr::key(bt[k]);
+              uint32_t b = first_slot<BS>(key, d.lg) / BS, n = 0;
+              while (true) {
+                key_t kk[BS];
+                bucket_keys(*reinterpret_cast<const uint4*>(&s_keys[b * BS]), kk);
+#pragma unroll
+                for (int i = 0; i < int(BS); ++i) n += kk[i] == key ? 1u : 0u;
+                if (b == bkt[k]) break;
+                b = (b + 1u) & bmask;
+              }
+              dup |= n > 1u;
+            }
+        }
       }
     }
     // Without duplicate keys in the chunk a probe has at most one match.  The
@@ -505,12 +539,11 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
     auto round_single = [&](const tup_t (&ptr)[R], uint32_t r0) {
       uint32_t slot[R];
       uint32_t walk = 0, found = 0;
+      walk = lanes_of(r0, d.shi, R);
 #pragma unroll
       for (int k = 0; k < R; ++k) {
-        bool v = r0 + uint32_t(k) * BLOCK + tid < d.shi;
-        if constexpr (!DIGIT_SENTINEL) v = v && KTr::key(ptr[k]) != sentinel;
-        walk |= v ? (1u << k) : 0u;
-        slot[k] = v ? first_slot<BS>(KTr::key(ptr[k]), d.lg) : 0u;
+        if constexpr (!DIGIT_SENTINEL) walk &= KTr::key(ptr[k]) != sentinel ? ~0u : ~(1u << k);
+        slot[k] = first_slot<BS>(KTr::key(ptr[k]), d.lg);
       }
       // The first step, taken by nearly every probe, without branches: the
       // bucket reads go out in groups of G (an invalid probe reads bucket 0
@@ -598,12 +631,11 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
     auto round_multi = [&](const tup_t (&ptr)[R], uint32_t r0) {
       uint32_t slot[R];
       uint32_t walk = 0;  // bit k: probe k has not reached its end
+      walk = lanes_of(r0, d.shi, R);
 #pragma unroll
       for (int k = 0; k < R; ++k) {
-        bool v = r0 + uint32_t(k) * BLOCK + tid < d.shi;
-        if constexpr (!DIGIT_SENTINEL) v = v && KTr::key(ptr[k]) != sentinel;
-        walk |= v ? (1u << k) : 0u;
-        slot[k] = v ? first_slot<BS>(KTr::key(ptr[k]), d.lg) : 0u;
+        if constexpr (!DIGIT_SENTINEL) walk &= KTr::key(ptr[k]) != sentinel ? ~0u : ~(1u << k);
+        slot[k] = first_slot<BS>(KTr::key(ptr[k]), d.lg);
       }
       while (true) {
         while (__ballot(walk != 0)) {
@@ -664,8 +696,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
       tup_t pr[R];
 #pragma unroll
       for (int k = 0; k < R; ++k) {
-        const uint32_t j = r0 + uint32_t(k) * BLOCK + tid;
-        if (j < d.shi) pr[k] = st[j];
+        if (r0 + uint32_t(k) * BLOCK < d.shi) pr[k] = st[min(r0 + uint32_t(k) * BLOCK + tid, d.shi - 1u)];
+        else pr[k] = tup_t{};
       }
       round(pr, r0);
     }
