@@ -234,13 +234,18 @@ def test_repeat_calls_reuse_workspace(pj, oracle, torch):
     assert np.array_equal(c, oracle.sort_pairs(oracle.inner_join_cpu(small_b.cpu().numpy(), small_p.cpu().numpy())))
 
 
-def test_config2_full_size_properties(pj, torch):
+@pytest.mark.parametrize("unique", [0, 1])
+def test_config2_full_size_properties(pj, torch, unique):
     """64M x 256M at full size, checked on the device through properties that
     do not need the CPU oracle: the pair count equals the number of probe keys
     below 2^26, every such probe row appears exactly once, and every pair joins
-    equal keys."""
+    equal keys.  With and without the plan's unique-key promise (the bench's
+    plan declares it)."""
     bk, pk = pj.generators.workload_config2()
-    pairs, n = pj.partitioned_join(bk, pk, return_count=True)
+    print(f"64M x 256M, unique_build_keys={unique}", flush=True)
+    plan = pj.plan_default(bk.numel(), pk.numel())  # the bench's plan: 8 + 8 bits, chunk factor 1.5
+    plan.unique_build_keys = unique
+    pairs, n = pj.partitioned_join(bk, pk, plan=plan, return_count=True)
     match = pk < (1 << 26)
     assert n == int(match.sum().item())
     b = pairs[:, 0].long()
