@@ -107,8 +107,9 @@ const char* pj_version(void);
 /* Message of the calling thread's last error ("" if none). */
 const char* pj_last_error(void);
 
-/* Plain plan for a join of these tables (reads only n and key_bytes): radix bits so the average build
- * partition fits the LDS table, 1 pass up to 2^11 partitions, else 2 passes.
+/* Plain plan for a join of these tables (reads only n and key_bytes): the fewest radix bits whose
+ * average build partition holds at most 2048 tuples (4-byte keys) or 1024 (8-byte keys), 1 pass
+ * up to 2^10 partitions, else 2 passes (ceil / floor halves of the bits).
  * Replaces the reference's hand-chosen -n/-N (main.cu:239-254). */
 pj_status pj_plan_default(const pj_table* build, const pj_table* probe, pj_plan* plan);
 
@@ -329,8 +330,9 @@ pj_status pj_planner_train_step(float* d_weights, float* d_adam_m, float* d_adam
  * T = |S| / 1024 probe rows.  The predicted heavy set (flag logit > 0)
  * disagrees with the exact counts if a candidate predicted heavy has fewer
  * than T / 2 rows or one not predicted heavy has 2T or more.  On any
- * disagreement, or if the plan's build chunks would not fit the LDS table,
- * the plain plan is returned and *fallback = 1. */
+ * disagreement, or if the plan's partitions are larger than the plain plan's
+ * aim and its build chunks would not fit the LDS table, the plain plan is
+ * returned and *fallback = 1. */
 pj_status pj_plan_decode(const float* logits, const uint32_t* cand_sample_counts, const uint64_t* cand_exact_counts,
                          const pj_table* build, const pj_table* probe, pj_plan* plan, int32_t* fallback);
 size_t pj_plan_learned_workspace_size(void);

@@ -153,13 +153,28 @@ static pj_status check_pair(const pj_table* b, const pj_table* p, bool need_data
 // 3.0 (main.cu:96) to absorb skew inside one block, here heavy partitions are
 // split on both sides instead, and the smaller table lets two join workgroups
 // share a CU's LDS.
+// Build tuples per partition, on average, that the plain plan aims at: 2048
+// for 4-byte keys, 1024 for 8-byte keys (build chunks hold at most
+// join_table_capacity / 2 - 1 tuples: 2783 and 2047).
+static double partition_target(int key_bytes) { return key_bytes == 4 ? 2048.0 : 1024.0; }
+
 static void plain_plan(int64_t nR, int key_bytes, pj_plan* plan) {
   std::memset(plan, 0, sizeof(*plan));
-  const double f = 1.5;
-  const double cap = double(join_table_capacity(key_bytes) / 2 - 1);
-  const double target = cap < 1536.0 ? cap : 1536.0;  // build tuples per chunk at factor f
+  // Build chunks of up to 1.25x the average partition: 2560 tuples at the
+  // 4-byte-key target, five per join thread (at 1.5x the chunk needs six, and
+  // that kernel's registers spill without the unique-key promise: 64M x 256M
+  // 3.81 against 3.68 ms).
+  const double f = 1.25;
+  // The fewest radix bits whose partitions hold at most partition_target()
+  // build tuples on average.  Fewer, larger
+  // partitions make the histogram, the offsets and the scatters cheaper than
+  // they make the join dearer while the chunks still fit: 64M x 256M int32
+  // 3.61 ms in 8 + 8 bits, 3.47 ms in 8 + 7; 16M x 256M Zipf(0.5) 3.21 ms in
+  // 7 + 7, 3.04 ms in 7 + 6; with 8-byte keys 2048-tuple partitions overflow
+  // the chunk (64M x 256M: 7.17 ms in 8 + 8, 8.43 ms in 8 + 7).
+  const double T = partition_target(key_bytes);
   int B = 1;
-  while (B < kMaxCombinedBits && f * double(nR) / double(int64_t(1) << B) > target) ++B;
+  while (B < kMaxCombinedBits && double(nR) / double(int64_t(1) << B) > T) ++B;
   plan->max_partition_size_factor = f;
   plan->unique_build_keys = 0;
   plan->probe_slice = 0;
@@ -915,10 +930,12 @@ static bool learned_plan_of(const float* logits, const pj_table* build, pj_plan*
     learned->bits2 = b1 + b2 > kMaxCombinedBits ? kMaxCombinedBits - b1 : b2;
   }
   learned->source = 1;
-  const int cap = join_table_capacity(build->key_bytes) / 2 - 1;
-  const double per = learned->max_partition_size_factor * double(build->n) /
-                     double(int64_t(1) << (learned->bits1 + learned->bits2));
-  return per <= double(cap) && check_plan(learned) == PJ_OK;
+  // The plan fits when its partitions are no larger than the plain plan's aim
+  // or its chunks (at the plan's factor) fit the table.
+  const double cap = double(join_table_capacity(build->key_bytes) / 2 - 1);
+  const double avg = double(build->n) / double(int64_t(1) << (learned->bits1 + learned->bits2));
+  const bool fits = avg <= partition_target(build->key_bytes) || learned->max_partition_size_factor * avg <= cap;
+  return fits && check_plan(learned) == PJ_OK;
 }
 
 static bool same_plan(const pj_plan& a, const pj_plan& b) {

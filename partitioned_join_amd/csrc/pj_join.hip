@@ -879,6 +879,7 @@ static void launch_join_kernel_t(const JoinArgs& A, int key_bytes, int base, int
   // build tuples per thread: NB * kJoinBlock >= base (base <= table_cap / 2 - 1)
   if (key_bytes == 4) {
     if (base <= 3 * kJoinBlock) join_launch_t<4, 3, kJoinProbe4, DS>(A, unique, lds, s);
+    else if (base <= 5 * kJoinBlock) join_launch_t<4, 5, kJoinProbe4, DS>(A, unique, lds, s);
     else join_launch_t<4, 6, kJoinProbe4, DS>(A, unique, lds, s);
   } else {
     if (base <= 2 * kJoinBlock) join_launch_t<8, 2, kJoinProbe8, DS>(A, unique, lds, s);

@@ -34,19 +34,25 @@ def test_version(pj):
 
 
 def test_plain_plans(pj):
+    # partitions of at most 2048 build rows on average for 4-byte keys, 1024 for 8-byte keys
     p1 = pj.plan_default(1 << 20, 1 << 22)
-    assert (p1.passes, p1.bits1, p1.bits2) == (1, 10, 0)
+    assert (p1.passes, p1.bits1, p1.bits2) == (1, 9, 0)
     p2 = pj.plan_default(1 << 26, 1 << 28)
-    assert (p2.passes, p2.bits1, p2.bits2) == (2, 8, 8)
+    assert (p2.passes, p2.bits1, p2.bits2) == (2, 8, 7)
     p3 = pj.plan_default(1 << 24, 1 << 28)
-    assert (p3.passes, p3.bits1 + p3.bits2) == (2, 14)
+    assert (p3.passes, p3.bits1, p3.bits2) == (2, 7, 6)
+    p4 = pj.plan_default(1 << 22, 1 << 24)
+    assert (p4.passes, p4.bits1, p4.bits2) == (2, 6, 5)
+    p8 = pj.plan_default(1 << 26, 1 << 28, 8)
+    assert (p8.passes, p8.bits1, p8.bits2) == (2, 8, 8)
+    assert (pj.plan_default((1 << 26) + 1, 1 << 28).bits1, pj.plan_default((1 << 26) + 1, 1 << 28).bits2) == (8, 8)
     tiny = pj.plan_default(5, 5)
     assert tiny.passes == 1 and tiny.bits1 >= 1
     huge = pj.plan_default((1 << 31) - 1, 1 << 20)
     assert huge.bits1 + huge.bits2 == 16
     for n in [1, 100, 10**4, 10**6, 10**7, 10**8]:
         p = pj.plan_default(n, n, 8)
-        assert 1 <= p.bits1 <= 12 and p.max_partition_size_factor == 1.5 and p.source == 0
+        assert 1 <= p.bits1 <= 12 and p.max_partition_size_factor == 1.25 and p.source == 0
         assert p.passes == (1 if p.bits2 == 0 else 2)
 
 

@@ -296,15 +296,15 @@ def test_plan_learned_agreement_fallback_and_same_pairs(pj, oracle, torch):
     a = oracle.sort_pairs(pj.partitioned_join(bk, pz, plan=plan2).cpu().numpy())
     b = oracle.sort_pairs(pj.partitioned_join(bk, pz, plan=plain).cpu().numpy())
     assert len(a) == pz.numel() and np.array_equal(a, b)
-    # weights predicting the plain plan itself (one pass of 10 bits) and no heavy key: there is
+    # weights predicting the plain plan itself (one pass of 9 bits) and no heavy key: there is
     # nothing for the heavy-set check to choose between, so the plan is taken without it
-    assert (plain.passes, plain.bits1) == (1, 10)
+    assert (plain.passes, plain.bits1) == (1, 9)
     b3 = np.full(O, -4.0, dtype=np.float32)
-    b3[10 - 6] = 4.0  # pass 1: 10 bits
+    b3[9 - 6] = 4.0   # pass 1: 9 bits
     b3[14 + 0] = 4.0  # one pass
     w[-O:] = torch.from_numpy(b3).cuda()
     plan3, fb3 = planner.plan_learned(bk, pz, w)
-    assert not fb3 and plan3.source == 1 and (plan3.passes, plan3.bits1, plan3.bits2) == (1, 10, 0)
+    assert not fb3 and plan3.source == 1 and (plan3.passes, plan3.bits1, plan3.bits2) == (1, 9, 0)
 
 
 @pytest.mark.gpu
