@@ -192,17 +192,16 @@ def autotune(builds, probes, base_plans: Sequence[PjPlan], chain_options=(4, 8, 
             for chains in chain_options:
                 bj = BatchJoin(builds, probes, plans, chains=chains)
                 ms = bj.time(reps)
+                bj.close()
                 key = (chains, delta) if tile == (None, None) else (chains, delta, tile)
                 timings[key] = ms
                 if log:
                     log(f"chains={chains} fanout{delta:+d} tile={tile}: {ms:.2f} ms")
                 if best is None or ms < best[0]:
-                    if best is not None:
-                        best[1].close()
-                    best = (ms, bj, key)
-                else:
-                    bj.close()
-    return best[1], best[2], timings
+                    best = (ms, plans, chains, key)
+    # Every timed batch is destroyed before the chosen configuration is built
+    # again: no returned graph lives through the destruction of another.
+    return BatchJoin(builds, probes, best[1], chains=best[2]), best[3], timings
 
 
 def make_batch_workload(count: int = 512, seed: int = 0xBA7C, device="cuda"):

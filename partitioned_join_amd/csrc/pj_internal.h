@@ -14,6 +14,7 @@ constexpr int kHistBlock = 1024;      // combined histogram workgroup
 constexpr int kScatterBlock = 1024;   // partition pass workgroup (one per CU: 16 waves)
 constexpr int kScatterTile4 = 16384;  // largest scatter tile, 4-byte keys (128 KB of LDS; scatter_cfg picks the tile per fan-out)
 constexpr int kScatterTile8 = 8192;   // 8-byte keys (128 KB)
+constexpr int kHeavyLanes = 8;        // lanes of a wave sharing one digit or bin from which ranks and counts are aggregated
 constexpr int kJoinBlock = 512;       // join workgroup
 constexpr int kJoinProbe4 = 9;        // probe tuples per thread and round, 4-byte keys (4608 per round)
 constexpr int kJoinProbe8 = 4;        // 8-byte keys (2048 per round)
