@@ -753,7 +753,7 @@ __host__ __device__ constexpr size_t scatter_lds_bytes(int F) {
 // Tile and carry of a scatter pass over F digits, so that stage, counts and
 // line images fit 160 KB of LDS beside pass 2's item prefix (E = TILE / 1024
 // per thread, a multiple of 4 in pass 1, whose loads are 16 bytes wide):
-//   4-byte keys  F <= 64: 16K + carry; F <= 256: 12K (pass 1) or 14K (pass 2)
+//   4-byte keys  F <= 128: 16K + carry; F <= 256: 12K (pass 1) or 14K (pass 2)
 //                + carry; 512: 8K + carry; 1024: 16K; more: 8K;
 //   8-byte keys  (16-byte tuples) F <= 64: 8K + carry (pass 2), F <= 128: 8K +
 //                carry (pass 1); F <= 256: 6K (pass 1) or 7K (pass 2) + carry;
@@ -767,6 +767,7 @@ inline ScatterCfg scatter_cfg(int key_bytes, int F, bool pass2, int F1 = 0) {
   if (pass2 && F > 2048 && F1 > 2048) return {key_bytes == 4 ? 7168 : 3072, 0};
   if (key_bytes == 4) {
     if (F <= 64) return {16384, 64};
+    if (F <= 128) return {16384, 128};
     if (F <= 256) return {pass2 ? 14336 : 12288, 256};
     if (F == 512) return {8192, 512};
     if (F == 1024) return {16384, 0};
@@ -979,6 +980,7 @@ static hipError_t pass1_launch(const Layout& L, hipStream_t s) {
   if (c.tile == T_ && c.cf == C_) return pass1_launch_t<KB, VEC, PAY0, PAY1, T_, C_>(L, s)
   if constexpr (KB == 4) {
     PJ_P1(16384, 64);
+    PJ_P1(16384, 128);
     PJ_P1(12288, 256);
     PJ_P1(8192, 512);
     PJ_P1(16384, 0);
@@ -1261,6 +1263,7 @@ hipError_t launch_pass2(const Layout& L, int key_bytes, hipStream_t s) {
   }
   if (key_bytes == 4) {
     PJ_PASS2(4, 16384, 64)
+    PJ_PASS2(4, 16384, 128)
     PJ_PASS2(4, 14336, 256)
     PJ_PASS2(4, 8192, 512)
     PJ_PASS2(4, 16384, 0)
