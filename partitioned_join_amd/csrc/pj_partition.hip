@@ -758,13 +758,19 @@ __host__ __device__ constexpr size_t scatter_lds_bytes(int F) {
 //   8-byte keys  (16-byte tuples) F <= 64: 8K + carry (pass 2), F <= 128: 8K +
 //                carry (pass 1); F <= 256: 6K (pass 1) or 7K (pass 2) + carry;
 //                512: 4K + carry; 1024: 8K; more: 4K.
-// Pass 2 of a deep plan with 4096 x 4096 partitions: the item prefix over
-// 4096 pass-1 partitions leaves room for 7K (4-byte keys) or 3K tuples.
+// Pass 2 after 4096 pass-1 partitions: the item prefix takes 16 KB, so 128,
+// 256 and 1024 digits drop to the next instantiated tile that fits (no carry
+// for 128 and 256 digits: 16K or 8K tuples; 1024 digits: 8K or 4K), and more
+// than 2048 digits leave room for 7K (4-byte keys) or 3K tuples.
 struct ScatterCfg {
   int tile, cf;
 };
 inline ScatterCfg scatter_cfg(int key_bytes, int F, bool pass2, int F1 = 0) {
-  if (pass2 && F > 2048 && F1 > 2048) return {key_bytes == 4 ? 7168 : 3072, 0};
+  if (pass2 && F1 > 2048) {
+    if (F > 2048) return {key_bytes == 4 ? 7168 : 3072, 0};
+    if (F > 64 && F <= 256) return {key_bytes == 4 ? 16384 : 8192, 0};
+    if (F == 1024) return {key_bytes == 4 ? 8192 : 4096, 0};
+  }
   if (key_bytes == 4) {
     if (F <= 64) return {16384, 64};
     if (F <= 128) return {16384, 128};

@@ -95,14 +95,16 @@ def test_batch_autotune_sequence_with_planned_joins(pj, driver):
 @pytest.mark.parametrize("kb", [4, 8])
 def test_every_plan_shape_on_small_and_ragged_joins(pj, driver, kb):
     """Every pass count and radix-bit split up to 24 bits (above 16 the pass-2
-    digits are counted after pass 1) on tables from 1 row to 300K rows: the
-    host carve-up and every launch stay inside the buffers."""
+    digits are counted after pass 1; every split with 11 or 12 pass-1 bits,
+    whose pass-2 item prefix takes the most LDS) on tables from 1 row to 300K
+    rows: the host carve-up and every launch stay inside the buffers and the
+    LDS."""
     sizes = [(1, 1), (1, 5000), (63, 64), (1000, 7), (4097, 20000), (100000, 300000), (0, 10), (10, 0)]
     cap = pj.lib().pj_join_chunk_capacity(kb)
     script = joins_block(sizes, kb)
     bid = 0
     shapes = [(1, b, 0) for b in range(1, 13)] + [(2, b1, b2) for b1 in range(1, 13) for b2 in range(1, 13)
-                                                  if b1 + b2 <= 16 or (b1 + b2) % 3 == 0]
+                                                  if b1 + b2 <= 16 or (b1 + b2) % 3 == 0 or b1 >= 11]
     for passes, b1, b2 in shapes:
         ok = all(1.5 * nb / 2 ** (b1 + b2) <= cap for nb, _ in sizes)
         if not ok:
