@@ -1029,30 +1029,41 @@ hipError_t launch_pass1(const Layout& L, int key_bytes, hipStream_t s) {
 //     whose per-digit counts k_hist2 takes over pass 1's output.
 // s_pre[r] counts the items before rank r, s_pre[RN] all of them.  Ends with
 // a barrier.
+//
+// With s_run (pass 2), the items are also cut into 8 runs of about equal
+// tuple counts, one per XCD: an item's start is taken as t(r) + g * n / ng
+// (t(r) the tuples of the ranks before r, n and ng its partition's tuples
+// and items), and s_run[0] / s_run[1] receive the first item whose start
+// reaches x * N / 8 / (x + 1) * N / 8 (N all tuples; x = xcd).  Equal item
+// counts per run would give the XCD that holds a heavy key's partition all
+// of its items, each up to 256 times the size of an ordinary one.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ uint32_t p2_rank_to_p1(uint32_t r, uint32_t PX) { return (r % PX) * kNumXcd + r / PX; }
 
 template <int BLOCK, bool DEEP>
 __device__ __forceinline__ void pass2_item_prefix(const PartTable& T, int F1, int b2, uint32_t chunk, uint32_t* s_pre,
-                                                  uint32_t* s_tmp) {
+                                                  uint32_t* s_tmp, uint32_t* s_tmp2 = nullptr, uint32_t xcd = 0,
+                                                  uint32_t* s_run = nullptr) {
   const uint32_t PX = uint32_t(F1 + kNumXcd - 1) / kNumXcd, RN = PX * kNumXcd;
   constexpr int RPT = (kMaxRadix + BLOCK - 1) / BLOCK;  // ranks per thread
   const int tid = threadIdx.x;
-  uint32_t ng[RPT], sum = 0;
+  uint32_t ng[RPT], nt[RPT], sum = 0, tsum = 0;
 #pragma unroll
   for (int i = 0; i < RPT; ++i) {
     const uint32_t r = uint32_t(tid) * RPT + i, p1 = r < RN ? p2_rank_to_p1(r, PX) : 0xFFFFFFFFu;
     ng[i] = 0;
+    nt[i] = 0;
     if (p1 < uint32_t(F1)) {
       if constexpr (DEEP) {
-        const uint32_t n = T.offc[p1 + 1] - T.offc[p1];
-        ng[i] = (n + chunk - 1) / chunk;
+        nt[i] = T.offc[p1 + 1] - T.offc[p1];
+        ng[i] = (nt[i] + chunk - 1) / chunk;
       } else {
-        const uint32_t n = T.off[(p1 + 1) << b2] - T.off[p1 << b2];
-        ng[i] = n == 0 ? 0u : min(T.nseg, max(1u, n / chunk));
+        nt[i] = T.off[(p1 + 1) << b2] - T.off[p1 << b2];
+        ng[i] = nt[i] == 0 ? 0u : min(T.nseg, max(1u, nt[i] / chunk));
       }
     }
     sum += ng[i];
+    tsum += nt[i];
   }
   uint32_t total;
   uint32_t run = block_exclusive_scan<BLOCK>(sum, s_tmp, &total);
@@ -1063,6 +1074,34 @@ __device__ __forceinline__ void pass2_item_prefix(const PartTable& T, int F1, in
     run += ng[i];
   }
   if (tid == 0) s_pre[RN] = total;
+  if (s_run != nullptr) {
+    // the run bounds: the rank r < RN with t(r) < A <= t(r) + n (the last
+    // rank if none) holds the first item whose start reaches A
+    uint32_t ttotal;
+    uint32_t trun = block_exclusive_scan<BLOCK>(tsum, s_tmp2, &ttotal);
+    uint32_t irun = run - sum;
+    const uint32_t A[2] = {uint32_t(uint64_t(xcd) * ttotal / kNumXcd),
+                           uint32_t(uint64_t(xcd + 1) * ttotal / kNumXcd)};
+#pragma unroll
+    for (int i = 0; i < RPT; ++i) {
+      const uint32_t r = uint32_t(tid) * RPT + i;
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        if (r < RN && trun < A[b] && (r + 1 == RN || trun + nt[i] >= A[b])) {
+          const uint64_t c = nt[i] == 0 ? 0ull : (uint64_t(A[b] - trun) * ng[i] + nt[i] - 1) / nt[i];
+          s_run[b] = irun + uint32_t(c < ng[i] ? c : ng[i]);
+        }
+      }
+      trun += nt[i];
+      irun += ng[i];
+    }
+    // no rank has t(r) < 0: a bound at A = 0 is item 0 (A = N > 0 finds the
+    // last non-empty rank, and with it all the items)
+    if (tid == 0) {
+      if (A[0] == 0) s_run[0] = 0;
+      if (A[1] == 0) s_run[1] = 0;
+    }
+  }
   lds_barrier();
 }
 
@@ -1100,22 +1139,24 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
   const uint32_t PX = uint32_t(F1 + kNumXcd - 1) / kNumXcd, RN = PX * kNumXcd;
   uint32_t* s_pre = reinterpret_cast<uint32_t*>(s_line + CF * Carry<KB>::LT);  // RN + 1 entries
   uint32_t* s_tmp = s_pre + RN + 1;   // BLOCK / 64 + 1 entries
+  uint32_t* s_tmp2 = s_tmp + BLOCK / kWave + 1;  // BLOCK / 64 + 1 entries
+  uint32_t* s_run = s_tmp2 + BLOCK / kWave + 1;  // 2 entries
   const int tid = threadIdx.x;
   const uint32_t chunk = uint32_t(kPass2ItemTiles) * uint32_t(TILE);
   auto p1_of = [&](uint32_t r) { return p2_rank_to_p1(r, PX); };
   for (int d = tid; d < F2; d += BLOCK) s_cnt[d] = 0;
-  pass2_item_prefix<BLOCK, DEEP>(T, F1, b2, chunk, s_pre, s_tmp);
+  pass2_item_prefix<BLOCK, DEEP>(T, F1, b2, chunk, s_pre, s_tmp, s_tmp2, uint32_t(x), s_run);
   // This workgroup's items (LDS values read as scalars: every lane holds the
-  // same): the items, in rank order, are cut into 8 equal runs and the
-  // workgroups of XCD x stride over run x.  With equal XCD shares run x is
-  // exactly the partitions p1 = x mod 8, so a partition's partly written
-  // output lines meet in one L2; a heavy partition's excess moves to the
-  // neighbouring XCDs (16M x 256M Zipf 1.0: pass 2 1.47 -> 1.19 ms, against
-  // 1.15 ms with no XCD placement at all, which costs 2.4 % on uniform keys).
+  // same): the items, in rank order, are cut into 8 runs of about equal tuple
+  // counts and the workgroups of XCD x stride over run x.  With equal XCD
+  // shares run x is exactly the partitions p1 = x mod 8, so a partition's
+  // partly written output lines meet in one L2; a heavy partition's excess
+  // moves to the neighbouring XCDs (16M x 256M Zipf 1.0: pass 2 1.47 -> 1.19
+  // ms, against 1.15 ms with no XCD placement at all, which costs 2.4 % on
+  // uniform keys).
   auto pre = [&](uint32_t r) { return uint32_t(__builtin_amdgcn_readfirstlane(s_pre[r])); };
-  const uint32_t n_items = pre(RN);
-  const uint32_t k_begin = uint32_t(x) * n_items / kNumXcd + uint32_t(j);
-  const uint32_t k_end = uint32_t(x + 1) * n_items / kNumXcd;
+  const uint32_t k_begin = uint32_t(__builtin_amdgcn_readfirstlane(s_run[0])) + uint32_t(j);
+  const uint32_t k_end = uint32_t(__builtin_amdgcn_readfirstlane(s_run[1]));
   auto swap_counts = [&]() {
     uint32_t* t = s_cnt;
     s_cnt = s_cnt_next;
@@ -1256,7 +1297,7 @@ hipError_t launch_pass2(const Layout& L, int key_bytes, hipStream_t s) {
   dim3 grid(ngrid), block(kScatterBlock);
   // the item prefix (one entry per pass-1 partition rounded up to 8, plus one)
   // and the scan's temporaries follow the scatter's LDS
-  const size_t extra = (size_t((L.pp.F1 + kNumXcd - 1) / kNumXcd) * kNumXcd + 1 + kScatterBlock / kWave + 1) * 4;
+  const size_t extra = (size_t((L.pp.F1 + kNumXcd - 1) / kNumXcd) * kNumXcd + 1 + 2 * (kScatterBlock / kWave + 1) + 2) * 4;
   const ScatterCfg c = scatter_cfg(key_bytes, L.pp.F2, true, L.pp.F1);
 #define PJ_PASS2(KB_, TILE_, CF_)                                                                                   \
   if (c.tile == TILE_ && c.cf == CF_) {                                                                             \

@@ -197,7 +197,10 @@ def test_int64_payload_columns_and_duplicates(pj, oracle, torch, plan):
 @pytest.mark.parametrize("plan", [{"passes": 2, "bits1": 9, "bits2": 9},
                                   {"passes": 2, "bits1": 12, "bits2": 5, "unique_build_keys": 1},
                                   {"passes": 2, "bits1": 6, "bits2": 12},
-                                  {"passes": 2, "bits1": 12, "bits2": 12, "max_partition_size_factor": 1.0}])
+                                  {"passes": 2, "bits1": 12, "bits2": 12, "max_partition_size_factor": 1.0},
+                                  {"passes": 2, "bits1": 12, "bits2": 7},
+                                  {"passes": 2, "bits1": 12, "bits2": 8, "unique_build_keys": 1},
+                                  {"passes": 2, "bits1": 12, "bits2": 10}])
 def test_deep_plans_beyond_the_combined_histogram(pj, oracle, torch, plan):
     """More than 2^16 partitions (the reference's skew script uses 512 x 512):
     pass-2 digits are counted per item after pass 1.  Unique and duplicate
