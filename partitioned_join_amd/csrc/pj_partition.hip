@@ -148,25 +148,21 @@ __global__ __launch_bounds__(kHistBlock) void k_hist(PartTable t0, PartTable t1,
     if (PACKED) atomicAdd(&s_bins[d >> 1], 1u << ((d & 1u) << 4));
     else atomicAdd(&s_bins[d], 1u);
   };
-  // kHeavyLanes or more active lanes in the first active lane's bin: one add
-  // for them (same-address LDS atomics serialise; Zipf(1.5) probes 0.42 ->
-  // 0.38 ms).  Used for a batch whose first key row looks heavy.
-  auto count_heavy = [&](uint32_t low) {
+  // The wave's hot bin (same-address LDS atomics serialise): when
+  // kHeavyLanes or more lanes of a batch's first key row fall in it, each key
+  // row of the batch counts its lanes in that bin with one add; otherwise the
+  // batch counts key by key and lane 0's first bin becomes the candidate for
+  // the next batch.  Under a heavy key the candidate settles on its bin
+  // within a few batches and keeps it, whichever lanes hold the key.
+  uint32_t hot = 0xFFFFFFFFu;
+  auto count_hot = [&](uint32_t low, uint32_t h) {
     const uint32_t d = (low >> pp.shift0) & pmask;
-    const uint32_t d0 = uint32_t(__builtin_amdgcn_readfirstlane(int(d)));
-    const uint64_t same = __ballot(d == d0);
-    const uint32_t ns = uint32_t(__popcll(same));
-    if (ns >= kHeavyLanes) {
-      if (d == d0) {
-        if (mbcnt(same) == 0u) {
-          if (PACKED) atomicAdd(&s_bins[d >> 1], ns << ((d & 1u) << 4));
-          else atomicAdd(&s_bins[d], ns);
-        }
-        return;
-      }
+    const uint64_t same = __ballot(d == h);
+    const uint32_t inc = d != h ? 1u : (mbcnt(same) == 0u ? uint32_t(__popcll(same)) : 0u);
+    if (inc) {
+      if (PACKED) atomicAdd(&s_bins[d >> 1], inc << ((d & 1u) << 4));
+      else atomicAdd(&s_bins[d], inc);
     }
-    if (PACKED) atomicAdd(&s_bins[d >> 1], 1u << ((d & 1u) << 4));
-    else atomicAdd(&s_bins[d], 1u);
   };
   // Batch f: rounds f * BATCH .. f * BATCH + BATCH - 1, all in range.
   auto load_batch = [&](uint32_t f, uint32_t (&low)[BATCH][4]) {
@@ -186,12 +182,13 @@ __global__ __launch_bounds__(kHistBlock) void k_hist(PartTable t0, PartTable t1,
   };
   auto count_low = [&](const uint32_t (&low)[BATCH][4]) {
     const uint32_t d = (low[0][0] >> pp.shift0) & pmask;
-    if (__popcll(__ballot(d == uint32_t(__builtin_amdgcn_readfirstlane(int(d))))) >= kHeavyLanes) {
+    if (__popcll(__ballot(d == hot)) >= kHeavyLanes) {
 #pragma unroll
       for (uint32_t u = 0; u < BATCH; ++u)
 #pragma unroll
-        for (int c = 0; c < 4; ++c) count_heavy(low[u][c]);
+        for (int c = 0; c < 4; ++c) count_hot(low[u][c], hot);
     } else {
+      hot = uint32_t(__builtin_amdgcn_readfirstlane(int(d)));
 #pragma unroll
       for (uint32_t u = 0; u < BATCH; ++u)
 #pragma unroll
