@@ -246,7 +246,7 @@ def test_config2_full_size_properties(pj, torch, unique):
     plan declares it)."""
     bk, pk = pj.generators.workload_config2()
     print(f"64M x 256M, unique_build_keys={unique}", flush=True)
-    plan = pj.plan_default(bk.numel(), pk.numel())  # the bench's plan: 8 + 8 bits, chunk factor 1.5
+    plan = pj.plan_default(bk.numel(), pk.numel())  # the bench's plan: 8 + 7 bits, chunk factor 1.25
     plan.unique_build_keys = unique
     pairs, n = pj.partitioned_join(bk, pk, plan=plan, return_count=True)
     match = pk < (1 << 26)
