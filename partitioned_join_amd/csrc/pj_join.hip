@@ -423,20 +423,30 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
     for (int k = 0; k < NB; ++k) tup_pin(bt[k]);
 #pragma unroll
     for (int k = 0; k < R; ++k) tup_pin(pt[k]);
-    tup_t bn[NB], pn[R];
-    if (dn.valid) load_item(dn, bn, pn);
-    const uint64_t item_nn = item + 2 * uint64_t(gridDim.x);
-    raw_desc_load(A, item_nn, ra, rb);  // decoded at the end of this item
     const uint32_t tsize = BS << d.lg, tmask = tsize - 1u;
-    key_t sentinel = key_t(d.q ^ 1u);
-    if constexpr (!DIGIT_SENTINEL) sentinel = chunk_sentinel(d);
-    {
-      const uint4 fill = bucket_fill(sentinel);
+    auto fill_table = [&](key_t empty) {
+      const uint4 fill = bucket_fill(empty);
       uint4* s4 = reinterpret_cast<uint4*>(s_keys);
       for (uint32_t i = tid; i < tsize / BS; i += BLOCK) {
         s4[i] = fill;
         s_cnt[i] = 0u;
       }
+    };
+    key_t sentinel = key_t(d.q ^ 1u);
+    tup_t bn[NB], pn[R];
+    const uint64_t item_nn = item + 2 * uint64_t(gridDim.x);
+    if constexpr (DIGIT_SENTINEL) {
+      // The table is filled before the next prefetch is issued: a register
+      // reload at the fill (the multi-match kernels run out of registers)
+      // then waits for nothing that is still in flight.
+      fill_table(sentinel);
+      if (dn.valid) load_item(dn, bn, pn);
+      raw_desc_load(A, item_nn, ra, rb);  // decoded at the end of this item
+    } else {
+      if (dn.valid) load_item(dn, bn, pn);
+      raw_desc_load(A, item_nn, ra, rb);
+      sentinel = chunk_sentinel(d);
+      fill_table(sentinel);
     }
     lds_barrier();
     // The previous item's matches go out now, after this item's prefetch was
