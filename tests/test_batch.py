@@ -116,6 +116,29 @@ def test_batch_graph_replays_match_eager(pj, oracle, torch):
 
 
 @pytest.mark.gpu
+def test_batch_back_to_back_replays_match_eager(pj, oracle, torch):
+    """Replays launched back to back without a host wait between them (as a
+    timing loop does), with the plans' own fan-out and with one more radix bit
+    on more chains: after the last replay every join's pairs equal the eager
+    join's, which the replay test checks against the oracle."""
+    from partitioned_join_amd import batch
+    builds, probes = _small_batch(torch, pj, seed=11)
+    expect = _eager(pj, oracle, builds, probes)
+    plain = batch.batch_plans(builds, probes, "plain")
+    shifted = [batch.shift_plan(p, 1, b.numel(), b.element_size()) for p, b in zip(plain, builds)]
+    for plans, chains in ((plain, 4), (shifted, 8)):
+        bj = batch.BatchJoin(builds, probes, plans, chains=chains)
+        for _ in range(4):
+            bj.launch()
+        torch.cuda.synchronize()
+        counts = bj.counts()
+        for j in range(len(builds)):
+            assert counts[j] == len(expect[j]), (chains, j)
+            assert np.array_equal(oracle.sort_pairs(bj.pairs(j, counts).cpu().numpy()), expect[j]), (chains, j)
+        bj.close()
+
+
+@pytest.mark.gpu
 @pytest.mark.skip(reason="the process aborted in pj_batch_launch (and once segfaulted checking the pairs) after "
                          "autotune on this batch with learned plans and an empty join; the cause is not found yet, so "
                          "the case stays off the GPU")
