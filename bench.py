@@ -44,7 +44,10 @@ def algorithmic_bytes(n_r, n_s, n_out, passes, w=8):
 
 
 def run_join_steps(pj, torch, bk, pk, plan, steps, warmup, bp=None, pp=None, n_cap=None):
-    """warmup + timed steps of the async join; returns (ms/step, phase means, count)."""
+    """warmup + timed steps of the async join, queued back to back; returns
+    (ms/step, phase means, count).  The phase means come from the HIP events
+    each timed join records (the last 128 when there are more), read after
+    the timed region."""
     n_cap = n_cap if n_cap is not None else pk.numel() + 1024
     out = torch.empty((n_cap, 2), dtype=torch.int32, device="cuda")
     ws_bytes = pj.workspace_size(bk.numel(), pk.numel(), bk.element_size(), plan)
@@ -53,14 +56,14 @@ def run_join_steps(pj, torch, bk, pk, plan, steps, warmup, bp=None, pp=None, n_c
         d_count = pj.partitioned_join_async(bk, pk, out, bp, pp, plan=plan, workspace=ws)
     torch.cuda.synchronize()
     pj.profile_enable(True)
-    phases = []
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
         d_count = pj.partitioned_join_async(bk, pk, out, bp, pp, plan=plan, workspace=ws)
-        phases.append(pj.profile_read())
     torch.cuda.synchronize()
     t1 = time.perf_counter()
+    phases = pj.profile_read_recent(steps)
+    assert len(phases) == min(steps, 128) and all(p["valid"] for p in phases), len(phases)
     pj.profile_enable(False)
     count = int(d_count.item())
     keys = [k for k in phases[0] if k.endswith("_ms")]

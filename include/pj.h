@@ -290,6 +290,12 @@ pj_status pj_gen_row_ids(int32_t* d_pay, int64_t n, hipStream_t stream);
 pj_status pj_profile_enable(int32_t on);
 /* Synchronises on the last recorded events and returns the phase times. */
 pj_status pj_profile_read(pj_phase_times* times);
+/* The phase times of the last min(n, 128, joins profiled since
+ * pj_profile_enable) profiled joins on this thread, oldest first, into
+ * times[0..*got), after one wait for the newest: joins queued back to back
+ * are timed without a host wait between them.  Joins with an empty table
+ * record no times. */
+pj_status pj_profile_read_recent(pj_phase_times* times, int32_t n, int32_t* got);
 
 /* ------------------------------------------------------------------------ */
 /* Learned planner.  No reference counterpart: the reference's plan is fixed  */

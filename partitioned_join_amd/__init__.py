@@ -26,7 +26,7 @@ __all__ = [
     "combined_histogram_pass", "histogram_pass", "exclusive_scan", "inclusive_scan",
     "partition_input_tables_pass_one", "partition_input_tables_pass_two", "handle_skewed_data",
     "shared_memory_hash_join", "project_reference", "hash_keys", "fastdiv", "tuple_keys", "tuple_payloads",
-    "profile_enable", "profile_read", "version", "lib",
+    "profile_enable", "profile_read", "profile_read_recent", "version", "lib",
 ]
 
 
@@ -363,6 +363,15 @@ def profile_read() -> dict:
     t = PjPhaseTimes()
     _lib.check(lib().pj_profile_read(ctypes.byref(t)))
     return t.as_dict()
+
+
+def profile_read_recent(n: int) -> list:
+    """Phase times of the last n profiled joins on this thread (at most 128),
+    oldest first, after one wait for the newest."""
+    times = (PjPhaseTimes * max(1, n))()
+    got = ctypes.c_int32(0)
+    _lib.check(lib().pj_profile_read_recent(times, int(n), ctypes.byref(got)))
+    return [times[i].as_dict() for i in range(got.value)]
 
 
 from . import generators  # noqa: E402,F401

@@ -241,17 +241,22 @@ static int64_t join_slice(const pj_plan* plan, int base) {
 // ---------------------------------------------------------------------------
 // Profiling
 // ---------------------------------------------------------------------------
+// A ring of event sets, one per profiled join, so that joins can be queued
+// back to back and their phase times read after one wait at the end.
+constexpr int kProfRing = 128;
 struct Prof {
   bool on = false;
-  bool recorded = false;
-  hipEvent_t ev[9] = {};
+  bool recorded = false;  // the last join on this thread recorded its events
+  int head = 0;           // the set the next join records into
+  int count = 0;          // recorded sets since profiling was enabled (at most kProfRing)
+  hipEvent_t ev[kProfRing][8] = {};
   bool created = false;
 };
 static thread_local Prof g_prof;
 
 static void prof_mark(int i, hipStream_t s) {
   if (!g_prof.on || !g_prof.created) return;
-  (void)hipEventRecord(g_prof.ev[i], s);
+  (void)hipEventRecord(g_prof.ev[g_prof.head][i], s);
 }
 
 // ---------------------------------------------------------------------------
@@ -291,7 +296,8 @@ static pj_status run_join(const pj_table* build, const pj_table* probe, const pj
   L.t[1].pay = probe->payload;
 
   if (g_prof.on && !g_prof.created) {
-    for (auto& e : g_prof.ev) PJ_HIP(hipEventCreate(&e));
+    for (auto& set : g_prof.ev)
+      for (auto& e : set) PJ_HIP(hipEventCreate(&e));
     g_prof.created = true;
   }
   prof_mark(0, s);
@@ -332,7 +338,11 @@ static pj_status run_join(const pj_table* build, const pj_table* probe, const pj
   prof_mark(6, s);
   PJ_HIP(launch_defrag(L, out, capacity, d_count, s));
   prof_mark(7, s);
-  if (g_prof.on) g_prof.recorded = true;
+  if (g_prof.on) {
+    g_prof.recorded = true;
+    g_prof.head = (g_prof.head + 1) % kProfRing;
+    g_prof.count = g_prof.count < kProfRing ? g_prof.count + 1 : kProfRing;
+  }
   return PJ_OK;
 }
 
@@ -813,16 +823,16 @@ pj_status pj_gen_row_ids(int32_t* d_pay, int64_t n, hipStream_t stream) {
 pj_status pj_profile_enable(int32_t on) {
   g_prof.on = on != 0;
   g_prof.recorded = false;
+  g_prof.count = 0;
   return PJ_OK;
 }
 
-pj_status pj_profile_read(pj_phase_times* t) {
-  if (!t) return fail(PJ_ERR_INVALID_ARG, "times is NULL");
+// Phase times of event set k, whose last event has completed.
+static pj_status prof_times(int k, pj_phase_times* t) {
   std::memset(t, 0, sizeof(*t));
-  if (!g_prof.on || !g_prof.recorded) return PJ_OK;
-  PJ_HIP(hipEventSynchronize(g_prof.ev[7]));
+  const hipEvent_t* ev = g_prof.ev[k];
   float ms[7];
-  for (int i = 0; i < 7; ++i) PJ_HIP(hipEventElapsedTime(&ms[i], g_prof.ev[i], g_prof.ev[i + 1]));
+  for (int i = 0; i < 7; ++i) PJ_HIP(hipEventElapsedTime(&ms[i], ev[i], ev[i + 1]));
   t->histogram_ms = ms[0];
   t->offsets_ms = ms[1];
   t->pass1_ms = ms[2];
@@ -830,8 +840,34 @@ pj_status pj_profile_read(pj_phase_times* t) {
   t->skew_ms = ms[4];
   t->join_ms = ms[5];
   t->compact_ms = ms[6];
-  PJ_HIP(hipEventElapsedTime(&t->total_ms, g_prof.ev[0], g_prof.ev[7]));
+  PJ_HIP(hipEventElapsedTime(&t->total_ms, ev[0], ev[7]));
   t->valid = 1;
+  return PJ_OK;
+}
+
+pj_status pj_profile_read(pj_phase_times* t) {
+  if (!t) return fail(PJ_ERR_INVALID_ARG, "times is NULL");
+  std::memset(t, 0, sizeof(*t));
+  if (!g_prof.on || !g_prof.recorded) return PJ_OK;
+  const int k = (g_prof.head + kProfRing - 1) % kProfRing;
+  PJ_HIP(hipEventSynchronize(g_prof.ev[k][7]));
+  return prof_times(k, t);
+}
+
+pj_status pj_profile_read_recent(pj_phase_times* times, int32_t n, int32_t* got) {
+  if (!times || !got) return fail(PJ_ERR_INVALID_ARG, "times or got is NULL");
+  if (n < 0) return fail(PJ_ERR_INVALID_ARG, "n must be >= 0");
+  *got = 0;
+  if (!g_prof.on) return PJ_OK;
+  const int m = n < g_prof.count ? n : g_prof.count;
+  if (m == 0) return PJ_OK;
+  const int newest = (g_prof.head + kProfRing - 1) % kProfRing;
+  PJ_HIP(hipEventSynchronize(g_prof.ev[newest][7]));
+  for (int j = 0; j < m; ++j) {
+    const pj_status st = prof_times((g_prof.head + kProfRing - m + j) % kProfRing, &times[j]);
+    if (st != PJ_OK) return st;
+  }
+  *got = m;
   return PJ_OK;
 }
 

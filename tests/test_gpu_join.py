@@ -313,3 +313,33 @@ def test_gather_columns(pj, oracle, torch):
     assert bool((kb == kp).all().item())
     e0, e1 = pj.gather(pairs[:0], d(bcol), d(pcol))
     assert e0.numel() == 0 and e1.numel() == 0
+
+
+def test_profile_reads_joins_queued_back_to_back(pj, torch):
+    """Phase times of joins queued without a host wait: one set per joined
+    pair of tables (an empty table records none), oldest first, no phase
+    negative and the phases summing to the total."""
+    bk, pk = pj.generators.workload_config2(scale_shift=6)
+    out = torch.empty((pk.numel() + 1024, 2), dtype=torch.int32, device="cuda")
+    ws = torch.empty(pj.workspace_size(bk.numel(), pk.numel(), 4), dtype=torch.uint8, device="cuda")
+    empty = torch.empty(0, dtype=torch.int32, device="cuda")
+    pj.profile_enable(True)
+    try:
+        for k in range(3):
+            pj.partitioned_join_async(bk, pk, out, workspace=ws)
+            if k == 1:
+                pj.partitioned_join_async(empty, pk, out, workspace=ws)
+        pj.partitioned_join_async(bk, pk, out, workspace=ws)
+        rec = pj.profile_read_recent(10)
+        assert len(rec) == 4
+        for r in rec:
+            phases = [r[k] for k in ("histogram_ms", "offsets_ms", "pass1_ms", "pass2_ms", "skew_ms", "join_ms",
+                                     "compact_ms")]
+            assert r["valid"] == 1 and all(v >= 0 for v in phases) and r["join_ms"] > 0
+            assert abs(sum(phases) - r["total_ms"]) <= 1e-3 * r["total_ms"] + 1e-3
+        last = pj.profile_read()
+        assert last["valid"] == 1 and last["total_ms"] == rec[-1]["total_ms"]
+        assert len(pj.profile_read_recent(2)) == 2
+    finally:
+        pj.profile_enable(False)
+    assert pj.profile_read_recent(4) == []
