@@ -290,6 +290,12 @@ pj_status pj_gen_row_ids(int32_t* d_pay, int64_t n, hipStream_t stream);
 pj_status pj_profile_enable(int32_t on);
 /* Synchronises on the last recorded events and returns the phase times. */
 pj_status pj_profile_read(pj_phase_times* times);
+/* The phase boundaries later profiled joins on this thread record: bit i
+ * (0..7) is the event before phase i (bit 7: after the last phase); default
+ * 0xFF.  Each event costs the GPU a few microseconds, so a timing loop may
+ * record only the two around the phase it times.  Phases without both of
+ * their events read 0, as does total_ms without events 0 and 7. */
+pj_status pj_profile_select(uint32_t marks);
 /* The phase times of the last min(n, 128, joins profiled since
  * pj_profile_enable) profiled joins on this thread, oldest first, into
  * times[0..*got), after one wait for the newest: joins queued back to back

@@ -26,7 +26,7 @@ __all__ = [
     "combined_histogram_pass", "histogram_pass", "exclusive_scan", "inclusive_scan",
     "partition_input_tables_pass_one", "partition_input_tables_pass_two", "handle_skewed_data",
     "shared_memory_hash_join", "project_reference", "hash_keys", "fastdiv", "tuple_keys", "tuple_payloads",
-    "profile_enable", "profile_read", "profile_read_recent", "version", "lib",
+    "profile_enable", "profile_read", "profile_read_recent", "profile_select", "version", "lib",
 ]
 
 
@@ -363,6 +363,12 @@ def profile_read() -> dict:
     t = PjPhaseTimes()
     _lib.check(lib().pj_profile_read(ctypes.byref(t)))
     return t.as_dict()
+
+
+def profile_select(marks: int = 0xFF) -> None:
+    """The phase boundaries later profiled joins record (bit i: the event
+    before phase i, bit 7 the last); phases without both events read 0."""
+    _lib.check(lib().pj_profile_select(int(marks)))
 
 
 def profile_read_recent(n: int) -> list:

@@ -89,6 +89,7 @@ SIGNATURES = {
     "pj_profile_enable": (c_i32, [c_i32]),
     "pj_profile_read": (c_i32, [ctypes.POINTER(PjPhaseTimes)]),
     "pj_profile_read_recent": (c_i32, [ctypes.POINTER(PjPhaseTimes), c_i32, ctypes.POINTER(c_i32)]),
+    "pj_profile_select": (c_i32, [c_u32]),
     "pj_planner_param_count": (c_size, []),
     "pj_join_chunk_capacity": (c_i32, [c_i32]),
     "pj_planner_scratch_bytes": (c_size, [c_i32]),

@@ -249,13 +249,15 @@ struct Prof {
   bool recorded = false;  // the last join on this thread recorded its events
   int head = 0;           // the set the next join records into
   int count = 0;          // recorded sets since profiling was enabled (at most kProfRing)
+  uint32_t marks = 0xFFu; // the phase boundaries recorded (bit i: event i)
   hipEvent_t ev[kProfRing][8] = {};
+  uint8_t set_marks[kProfRing] = {};  // the boundaries each set recorded
   bool created = false;
 };
 static thread_local Prof g_prof;
 
 static void prof_mark(int i, hipStream_t s) {
-  if (!g_prof.on || !g_prof.created) return;
+  if (!g_prof.on || !g_prof.created || !((g_prof.marks >> i) & 1u)) return;
   (void)hipEventRecord(g_prof.ev[g_prof.head][i], s);
 }
 
@@ -340,6 +342,7 @@ static pj_status run_join(const pj_table* build, const pj_table* probe, const pj
   prof_mark(7, s);
   if (g_prof.on) {
     g_prof.recorded = true;
+    g_prof.set_marks[g_prof.head] = uint8_t(g_prof.marks);
     g_prof.head = (g_prof.head + 1) % kProfRing;
     g_prof.count = g_prof.count < kProfRing ? g_prof.count + 1 : kProfRing;
   }
@@ -827,12 +830,28 @@ pj_status pj_profile_enable(int32_t on) {
   return PJ_OK;
 }
 
-// Phase times of event set k, whose last event has completed.
+pj_status pj_profile_select(uint32_t marks) {
+  if (marks == 0 || marks > 0xFFu) return fail(PJ_ERR_INVALID_ARG, "marks must be a non-empty subset of 0xFF");
+  g_prof.marks = marks;
+  return PJ_OK;
+}
+
+// The last event that set k recorded.
+static hipEvent_t prof_last(int k) {
+  int i = 7;
+  while (i > 0 && !((g_prof.set_marks[k] >> i) & 1u)) --i;
+  return g_prof.ev[k][i];
+}
+
+// Phase times of event set k, whose last event has completed: a phase whose
+// two boundaries were not both recorded reads 0.
 static pj_status prof_times(int k, pj_phase_times* t) {
   std::memset(t, 0, sizeof(*t));
   const hipEvent_t* ev = g_prof.ev[k];
-  float ms[7];
-  for (int i = 0; i < 7; ++i) PJ_HIP(hipEventElapsedTime(&ms[i], ev[i], ev[i + 1]));
+  const uint32_t m = g_prof.set_marks[k];
+  float ms[7] = {};
+  for (int i = 0; i < 7; ++i)
+    if (((m >> i) & 3u) == 3u) PJ_HIP(hipEventElapsedTime(&ms[i], ev[i], ev[i + 1]));
   t->histogram_ms = ms[0];
   t->offsets_ms = ms[1];
   t->pass1_ms = ms[2];
@@ -840,7 +859,7 @@ static pj_status prof_times(int k, pj_phase_times* t) {
   t->skew_ms = ms[4];
   t->join_ms = ms[5];
   t->compact_ms = ms[6];
-  PJ_HIP(hipEventElapsedTime(&t->total_ms, ev[0], ev[7]));
+  if ((m & 0x81u) == 0x81u) PJ_HIP(hipEventElapsedTime(&t->total_ms, ev[0], ev[7]));
   t->valid = 1;
   return PJ_OK;
 }
@@ -850,7 +869,7 @@ pj_status pj_profile_read(pj_phase_times* t) {
   std::memset(t, 0, sizeof(*t));
   if (!g_prof.on || !g_prof.recorded) return PJ_OK;
   const int k = (g_prof.head + kProfRing - 1) % kProfRing;
-  PJ_HIP(hipEventSynchronize(g_prof.ev[k][7]));
+  PJ_HIP(hipEventSynchronize(prof_last(k)));
   return prof_times(k, t);
 }
 
@@ -862,7 +881,7 @@ pj_status pj_profile_read_recent(pj_phase_times* times, int32_t n, int32_t* got)
   const int m = n < g_prof.count ? n : g_prof.count;
   if (m == 0) return PJ_OK;
   const int newest = (g_prof.head + kProfRing - 1) % kProfRing;
-  PJ_HIP(hipEventSynchronize(g_prof.ev[newest][7]));
+  PJ_HIP(hipEventSynchronize(prof_last(newest)));
   for (int j = 0; j < m; ++j) {
     const pj_status st = prof_times((g_prof.head + kProfRing - m + j) % kProfRing, &times[j]);
     if (st != PJ_OK) return st;

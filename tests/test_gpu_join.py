@@ -340,6 +340,14 @@ def test_profile_reads_joins_queued_back_to_back(pj, torch):
         last = pj.profile_read()
         assert last["valid"] == 1 and last["total_ms"] == rec[-1]["total_ms"]
         assert len(pj.profile_read_recent(2)) == 2
+        # only the two events around the join phase: the other phases read 0
+        pj.profile_select(0x60)
+        pj.partitioned_join_async(bk, pk, out, workspace=ws)
+        sel = pj.profile_read_recent(1)[0]
+        assert sel["valid"] == 1 and sel["join_ms"] > 0
+        assert sel["histogram_ms"] == 0 and sel["pass1_ms"] == 0 and sel["total_ms"] == 0
+        assert len(pj.profile_read_recent(10)) == 5  # the earlier sets keep their own events
     finally:
+        pj.profile_select(0xFF)
         pj.profile_enable(False)
     assert pj.profile_read_recent(4) == []
