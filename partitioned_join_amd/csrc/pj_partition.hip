@@ -109,6 +109,13 @@ __device__ __forceinline__ void load4_low(const void* keys, uint32_t i, uint32_t
   }
 }
 
+// Sum over the 64 lanes of a wave (all lanes active), in every lane.
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
+#pragma unroll
+  for (int o = kWave / 2; o > 0; o >>= 1) v += uint32_t(__shfl_xor(int(v), o, kWave));
+  return v;
+}
+
 // ---------------------------------------------------------------------------
 // k_hist: per-segment fine histogram.
 //   PACKED == false: P <= 32768 u32 LDS bins.
@@ -221,7 +228,16 @@ __global__ __launch_bounds__(kHistBlock) void k_hist(PartTable t0, PartTable t1,
     for (int i = tid; i < P; i += BLOCK) {
       const uint32_t v = s_bins[i];
       row[i] = v;
-      if (two_pass && v) atomicAdd(&s_coarse[i >> b2], v);
+      if (two_pass) {
+        if (b2 >= 6) {
+          // a wave's 64 consecutive bins share one pass-1 bin: one add for
+          // them (64 same-address LDS atomics serialise)
+          const uint32_t w = wave_sum(v);
+          if (lane_id() == 0 && w) atomicAdd(&s_coarse[i >> b2], w);
+        } else if (v) {
+          atomicAdd(&s_coarse[i >> b2], v);
+        }
+      }
     }
   } else {
     constexpr int WPT = 32;  // LDS words per thread: 32768 words / 1024 threads
@@ -276,8 +292,13 @@ __global__ __launch_bounds__(kHistBlock) void k_hist(PartTable t0, PartTable t1,
       row[tid + BLOCK * k] = make_uint2(acc[2 * k], acc[2 * k + 1]);
       if (two_pass) {
         const uint32_t bin = 2u * (tid + BLOCK * k);
-        if (acc[2 * k]) atomicAdd(&s_coarse[bin >> b2], acc[2 * k]);
-        if (acc[2 * k + 1]) atomicAdd(&s_coarse[(bin + 1) >> b2], acc[2 * k + 1]);
+        if (b2 >= 7) {  // a wave's 128 consecutive bins share one pass-1 bin
+          const uint32_t w = wave_sum(acc[2 * k] + acc[2 * k + 1]);
+          if (lane_id() == 0 && w) atomicAdd(&s_coarse[bin >> b2], w);
+        } else {
+          if (acc[2 * k]) atomicAdd(&s_coarse[bin >> b2], acc[2 * k]);
+          if (acc[2 * k + 1]) atomicAdd(&s_coarse[(bin + 1) >> b2], acc[2 * k + 1]);
+        }
       }
     }
   }
